@@ -1,0 +1,82 @@
+"""Build recipes for the native libraries (host C++, HIP for gfx950, oracle).
+
+Every library is built in-tree so that it travels with the repository:
+
+  gicops_amd/lib/libgicops_host.so  index construction + synthetic data (g++/OpenMP)
+  gicops_amd/lib/libgicops_hip.so   HIP kernels + the C ABI of include/gicops_amd.h
+  oracle/liboracle.so               CPU restatement used only by tests and the
+                                    CPU baseline (built from oracle/Makefile)
+
+Floating-point contraction is off everywhere: the reference is built for
+baseline x86-64 (no FMA) and parity of float bins and hyperscores depends on
+unfused multiply/add.
+"""
+from __future__ import annotations
+
+import os
+import shutil
+import subprocess
+import sys
+from pathlib import Path
+
+ROOT = Path(__file__).resolve().parent.parent
+PKG = ROOT / "gicops_amd"
+CSRC = PKG / "csrc"
+LIBDIR = PKG / "lib"
+INCLUDE = ROOT / "include"
+ORACLE = ROOT / "oracle"
+
+HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+OFFLOAD_ARCH = "gfx950"
+
+HOST_SOURCES = [CSRC / "host" / "slm_build.cpp"]
+HIP_SOURCES = [
+    CSRC / "hip" / "search.hip",
+    CSRC / "hip" / "capi.hip",
+]
+HIP_HEADERS = sorted((CSRC / "hip").glob("*.h")) + sorted(INCLUDE.glob("*.h"))
+
+
+def _run(cmd: list[str], cwd: Path | None = None) -> None:
+    print("+", " ".join(str(c) for c in cmd), flush=True)
+    subprocess.run([str(c) for c in cmd], check=True, cwd=cwd)
+
+
+def build_host() -> Path:
+    LIBDIR.mkdir(parents=True, exist_ok=True)
+    out = LIBDIR / "libgicops_host.so"
+    tmp = out.with_suffix(".so.tmp")
+    _run(["g++", "-std=c++17", "-O3", "-fopenmp", "-fPIC", "-shared", "-ffp-contract=off",
+          "-Wall", "-Wextra", "-Wno-unused-parameter",
+          f"-I{INCLUDE}", f"-I{CSRC / 'host'}", *HOST_SOURCES, "-o", tmp])
+    os.replace(tmp, out)
+    return out
+
+
+def build_hip() -> Path:
+    LIBDIR.mkdir(parents=True, exist_ok=True)
+    out = LIBDIR / "libgicops_hip.so"
+    tmp = out.with_suffix(".so.tmp")
+    _run([HIPCC, f"--offload-arch={OFFLOAD_ARCH}", "-O3", "-std=c++17", "-fPIC", "-shared",
+          "-ffp-contract=off", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+          "-Wall", "-Wno-unused-parameter", "-Wno-unused-command-line-argument",
+          f"-I{INCLUDE}", f"-I{CSRC / 'hip'}", *HIP_SOURCES, "-o", tmp])
+    os.replace(tmp, out)
+    return out
+
+
+def build_oracle() -> Path:
+    _run(["make", "-C", str(ORACLE), "-B"])
+    return ORACLE / "liboracle.so"
+
+
+def build_all() -> None:
+    build_host()
+    build_oracle()
+    build_hip()
+
+
+if __name__ == "__main__":
+    what = sys.argv[1:] or ["all"]
+    for w in what:
+        {"host": build_host, "hip": build_hip, "oracle": build_oracle, "all": build_all}[w]()

@@ -1,0 +1,402 @@
+// C ABI of include/gicops_amd.h: contexts, device-resident index and query
+// batches, and the search entry points. No function here exits or throws
+// across the ABI; HIP failures map to GX_ERR_DEVICE.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <mutex>
+#include <new>
+#include <vector>
+
+#include "gicops_amd.h"
+#include "search_common.h"
+
+using namespace gx;
+
+struct gx_ctx {
+    int device = 0;
+    int n_cu = 0;
+    char name[256] = {};
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    double *d_lgfact = nullptr;
+    int lgfact_n = 0;
+    uint32_t *d_counter = nullptr;
+    unsigned long long *d_stats = nullptr;
+    // per-block scratch, grown on demand
+    uint32_t *d_slices = nullptr;
+    size_t slices_words = 0;
+    uint32_t *d_cand = nullptr;
+    size_t cand_words = 0;
+    std::mutex mu;
+};
+
+struct gx_dindex {
+    gx_ctx *ctx = nullptr;
+    int n_buckets = 0;
+    int maxz = 1;
+    uint32_t nbins = 0;
+    uint32_t max_speclen = 0;
+    std::vector<void *> allocs;
+    DevBucket *d_buckets = nullptr;
+    std::vector<DevBucket> h_buckets;
+};
+
+struct gx_dqueries {
+    gx_ctx *ctx = nullptr;
+    uint32_t n_spectra = 0;
+    uint64_t n_peaks = 0;
+    uint32_t max_peaks = 0;
+    bool has_dm = false;
+    uint32_t *idx = nullptr;
+    int32_t *moz = nullptr;
+    int32_t *inten = nullptr;
+    float *prec = nullptr;
+    float *dm = nullptr;
+    gx_psm *psm = nullptr;
+    uint32_t *hist = nullptr;  // allocated when histograms are requested
+    bool searched = false;
+};
+
+namespace {
+
+#define GX_HIP(call)                                   \
+    do {                                               \
+        hipError_t e_ = (call);                        \
+        if (e_ != hipSuccess) return GX_ERR_DEVICE;    \
+    } while (0)
+
+template <class T>
+int dalloc(T **p, size_t n) {
+    *p = nullptr;
+    if (n == 0) n = 1;
+    if (hipMalloc((void **)p, n * sizeof(T)) != hipSuccess) {
+        *p = nullptr;
+        return GX_ERR_BAD_MEM_ALLOC;
+    }
+    return GX_SUCCESS;
+}
+
+int ensure_scratch(gx_ctx *c, int n_blocks, uint32_t max_slices) {
+    const size_t sw = (size_t)n_blocks * 2 * max_slices;
+    if (sw > c->slices_words) {
+        if (c->d_slices) (void)hipFree(c->d_slices);
+        c->d_slices = nullptr;
+        c->slices_words = 0;
+        int st = dalloc(&c->d_slices, sw);
+        if (st) return st;
+        c->slices_words = sw;
+    }
+    const size_t cw = (size_t)n_blocks * kTileEntries;
+    if (cw > c->cand_words) {
+        if (c->d_cand) (void)hipFree(c->d_cand);
+        c->d_cand = nullptr;
+        c->cand_words = 0;
+        int st = dalloc(&c->d_cand, cw);
+        if (st) return st;
+        c->cand_words = cw;
+    }
+    return GX_SUCCESS;
+}
+
+int check_params(const gx_search_params *p, const gx_dindex *idx) {
+    if (p->dF < 0 || p->dF > 64 || p->scale < 1 || p->max_mass < 1 || p->min_shp < 0 || p->min_cpsm < 0)
+        return GX_ERR_INVLD_PARAM;
+    if ((uint64_t)p->max_mass * (uint64_t)p->scale != idx->nbins) return GX_ERR_INVLD_INDEX;
+    if (p->evalue_mode != GX_EVALUE_TAILFIT && p->evalue_mode != GX_EVALUE_LEARNED) return GX_ERR_INVLD_PARAM;
+    return GX_SUCCESS;
+}
+
+}  // namespace
+
+extern "C" int gx_ctx_create(int device, gx_ctx **out) {
+    if (!out) return GX_ERR_INVLD_PTR;
+    *out = nullptr;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return GX_ERR_NO_DEVICE;
+    if (device < 0 || device >= ndev) return GX_ERR_INVLD_PARAM;
+    gx_ctx *c = new (std::nothrow) gx_ctx();
+    if (!c) return GX_ERR_BAD_MEM_ALLOC;
+    c->device = device;
+    auto fail = [&](int st) {
+        gx_ctx_destroy(c);
+        return st;
+    };
+    if (hipSetDevice(device) != hipSuccess) return fail(GX_ERR_DEVICE);
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, device) != hipSuccess) return fail(GX_ERR_DEVICE);
+    if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0) return fail(GX_ERR_NO_DEVICE);
+    c->n_cu = prop.multiProcessorCount;
+    std::snprintf(c->name, sizeof(c->name), "%s (%s)", prop.name, prop.gcnArchName);
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) return fail(GX_ERR_DEVICE);
+    if (hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess) return fail(GX_ERR_DEVICE);
+    // lgfact[i] = sum_{j=2..i} log10(j) (utils.h:115-131), computed on the host
+    // with the same libm the CPU search uses, long enough for any count
+    c->lgfact_n = 4096;
+    std::vector<double> lg(c->lgfact_n);
+    lg[0] = std::log10(1.0);
+    lg[1] = std::log10(1.0);
+    for (int i = 2; i < c->lgfact_n; i++) lg[i] = std::log10((double)i) + lg[i - 1];
+    if (dalloc(&c->d_lgfact, lg.size())) return fail(GX_ERR_BAD_MEM_ALLOC);
+    if (dalloc(&c->d_counter, 1) || dalloc(&c->d_stats, ST_N)) return fail(GX_ERR_BAD_MEM_ALLOC);
+    if (hipMemcpy(c->d_lgfact, lg.data(), lg.size() * sizeof(double), hipMemcpyHostToDevice) != hipSuccess)
+        return fail(GX_ERR_DEVICE);
+    *out = c;
+    return GX_SUCCESS;
+}
+
+extern "C" void gx_ctx_destroy(gx_ctx *c) {
+    if (!c) return;
+    (void)hipSetDevice(c->device);
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    if (c->d_lgfact) (void)hipFree(c->d_lgfact);
+    if (c->d_counter) (void)hipFree(c->d_counter);
+    if (c->d_stats) (void)hipFree(c->d_stats);
+    if (c->d_slices) (void)hipFree(c->d_slices);
+    if (c->d_cand) (void)hipFree(c->d_cand);
+    if (c->ev0) (void)hipEventDestroy(c->ev0);
+    if (c->ev1) (void)hipEventDestroy(c->ev1);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+}
+
+extern "C" int gx_ctx_device_info(gx_ctx *c, char *name, size_t name_len, int *n_cu) {
+    if (!c) return GX_ERR_INVLD_PTR;
+    if (name && name_len) std::snprintf(name, name_len, "%s", c->name);
+    if (n_cu) *n_cu = c->n_cu;
+    return GX_SUCCESS;
+}
+
+extern "C" int gx_index_upload(gx_ctx *c, const gx_bucket *b, int nb, int maxz, gx_dindex **out) {
+    if (!c || !out || (nb > 0 && !b)) return GX_ERR_INVLD_PTR;
+    *out = nullptr;
+    if (nb < 1 || nb > kMaxBuckets || maxz < 1) return GX_ERR_INVLD_PARAM;
+    std::lock_guard<std::mutex> lk(c->mu);
+    if (hipSetDevice(c->device) != hipSuccess) return GX_ERR_DEVICE;
+    gx_dindex *d = new (std::nothrow) gx_dindex();
+    if (!d) return GX_ERR_BAD_MEM_ALLOC;
+    d->ctx = c;
+    d->n_buckets = nb;
+    d->maxz = maxz;
+    d->h_buckets.resize(nb);
+    auto fail = [&](int st) {
+        for (void *p : d->allocs) (void)hipFree(p);
+        if (d->d_buckets) (void)hipFree(d->d_buckets);
+        delete d;
+        return st;
+    };
+    uint32_t nbins = 0;
+    for (int i = 0; i < nb; i++) {
+        const gx_bucket &x = b[i];
+        if (x.n_entries == 0) continue;
+        if (!x.mass || !x.bA || !x.iA) return fail(GX_ERR_INVLD_PTR);
+        if (nbins == 0) nbins = x.nbins;
+        if (x.nbins != nbins || x.nbins < 2) return fail(GX_ERR_INVLD_INDEX);
+        if (x.speclen != (uint32_t)((x.peplen - 1) * maxz * 2) || x.speclen < 2) return fail(GX_ERR_INVLD_INDEX);
+        if ((uint64_t)x.n_entries * x.speclen != x.n_ions || x.n_ions >= (1ull << 32)) return fail(GX_ERR_INVLD_SIZE);
+        if (x.bA[x.nbins] != x.n_ions) return fail(GX_ERR_INVLD_INDEX);
+    }
+    d->nbins = nbins;
+    for (int i = 0; i < nb; i++) {
+        const gx_bucket &x = b[i];
+        DevBucket db{};
+        db.n_entries = x.n_entries;
+        db.speclen = x.speclen;
+        if (x.n_entries > 0) {
+            float *m;
+            uint32_t *bA, *iA, *ib;
+            if (dalloc(&m, x.n_entries) || dalloc(&bA, (size_t)x.nbins + 1) || dalloc(&iA, x.n_ions) ||
+                dalloc(&ib, x.n_ions)) {
+                return fail(GX_ERR_BAD_MEM_ALLOC);
+            }
+            d->allocs.push_back(m);
+            d->allocs.push_back(bA);
+            d->allocs.push_back(iA);
+            d->allocs.push_back(ib);
+            if (hipMemcpy(m, x.mass, sizeof(float) * x.n_entries, hipMemcpyHostToDevice) != hipSuccess ||
+                hipMemcpy(bA, x.bA, sizeof(uint32_t) * ((size_t)x.nbins + 1), hipMemcpyHostToDevice) != hipSuccess ||
+                hipMemcpy(iA, x.iA, sizeof(uint32_t) * x.n_ions, hipMemcpyHostToDevice) != hipSuccess)
+                return fail(GX_ERR_DEVICE);
+            if (launch_ion_bin(bA, iA, ib, x.nbins, x.n_ions, c->stream) != hipSuccess) return fail(GX_ERR_DEVICE);
+            db.mass = m;
+            db.bA = bA;
+            db.iA = iA;
+            db.ion_bin = ib;
+            d->max_speclen = std::max(d->max_speclen, x.speclen);
+        }
+        d->h_buckets[i] = db;
+    }
+    if (dalloc(&d->d_buckets, nb)) return fail(GX_ERR_BAD_MEM_ALLOC);
+    if (hipMemcpy(d->d_buckets, d->h_buckets.data(), sizeof(DevBucket) * nb, hipMemcpyHostToDevice) != hipSuccess)
+        return fail(GX_ERR_DEVICE);
+    if (hipStreamSynchronize(c->stream) != hipSuccess) return fail(GX_ERR_DEVICE);
+    *out = d;
+    return GX_SUCCESS;
+}
+
+extern "C" void gx_index_free(gx_dindex *d) {
+    if (!d) return;
+    (void)hipSetDevice(d->ctx->device);
+    (void)hipStreamSynchronize(d->ctx->stream);
+    for (void *p : d->allocs) (void)hipFree(p);
+    if (d->d_buckets) (void)hipFree(d->d_buckets);
+    delete d;
+}
+
+extern "C" int gx_queries_upload(gx_ctx *c, const gx_query_batch *q, gx_dqueries **out) {
+    if (!c || !q || !out) return GX_ERR_INVLD_PTR;
+    *out = nullptr;
+    if (q->n_spectra > 0 && (!q->idx || !q->precurse)) return GX_ERR_INVLD_PTR;
+    if (q->n_peaks > 0 && (!q->moz || !q->intensity)) return GX_ERR_INVLD_PTR;
+    if (q->n_spectra > 0 && q->idx[q->n_spectra] != q->n_peaks) return GX_ERR_INVLD_SIZE;
+    if (q->n_peaks >= (1ull << 32)) return GX_ERR_INVLD_SIZE;
+    uint32_t maxp = 0;
+    for (uint32_t s = 0; s < q->n_spectra; s++) {
+        if (q->idx[s + 1] < q->idx[s]) return GX_ERR_INVLD_SIZE;
+        maxp = std::max(maxp, q->idx[s + 1] - q->idx[s]);
+    }
+    if (maxp > (uint32_t)kMaxPeaksLds) return GX_ERR_INVLD_SIZE;
+    std::lock_guard<std::mutex> lk(c->mu);
+    if (hipSetDevice(c->device) != hipSuccess) return GX_ERR_DEVICE;
+    gx_dqueries *d = new (std::nothrow) gx_dqueries();
+    if (!d) return GX_ERR_BAD_MEM_ALLOC;
+    d->ctx = c;
+    d->n_spectra = q->n_spectra;
+    d->n_peaks = q->n_peaks;
+    d->max_peaks = maxp;
+    d->has_dm = q->dm != nullptr;
+    auto fail = [&](int st) {
+        gx_queries_free(d);
+        return st;
+    };
+    const size_t ns = q->n_spectra, np = q->n_peaks;
+    if (dalloc(&d->idx, ns + 1) || dalloc(&d->moz, np) || dalloc(&d->inten, np) || dalloc(&d->prec, ns) ||
+        dalloc(&d->psm, ns) || (d->has_dm && dalloc(&d->dm, ns)))
+        return fail(GX_ERR_BAD_MEM_ALLOC);
+    if (ns) {
+        if (hipMemcpy(d->idx, q->idx, (ns + 1) * 4, hipMemcpyHostToDevice) != hipSuccess ||
+            hipMemcpy(d->prec, q->precurse, ns * 4, hipMemcpyHostToDevice) != hipSuccess ||
+            (d->has_dm && hipMemcpy(d->dm, q->dm, ns * 4, hipMemcpyHostToDevice) != hipSuccess))
+            return fail(GX_ERR_DEVICE);
+    }
+    if (np) {
+        if (hipMemcpy(d->moz, q->moz, np * 4, hipMemcpyHostToDevice) != hipSuccess ||
+            hipMemcpy(d->inten, q->intensity, np * 4, hipMemcpyHostToDevice) != hipSuccess)
+            return fail(GX_ERR_DEVICE);
+    }
+    *out = d;
+    return GX_SUCCESS;
+}
+
+extern "C" void gx_queries_free(gx_dqueries *d) {
+    if (!d) return;
+    (void)hipSetDevice(d->ctx->device);
+    (void)hipStreamSynchronize(d->ctx->stream);
+    void *ps[] = {d->idx, d->moz, d->inten, d->prec, d->dm, d->psm, d->hist};
+    for (void *p : ps)
+        if (p) (void)hipFree(p);
+    delete d;
+}
+
+static int search_locked(gx_ctx *c, const gx_dindex *idx, gx_dqueries *q, const gx_search_params *p,
+                         gx_search_stats *stats, bool want_hist) {
+    int st = check_params(p, idx);
+    if (st) return st;
+    if (idx->ctx != c || q->ctx != c) return GX_ERR_INVLD_PARAM;
+    if (hipSetDevice(c->device) != hipSuccess) return GX_ERR_DEVICE;
+    if (want_hist && !q->hist) {
+        st = dalloc(&q->hist, (size_t)q->n_spectra * GX_HIST_BINS);
+        if (st) return st;
+    }
+    const uint32_t nb5 = (uint32_t)(2 * p->dF + 1);
+    const uint32_t max_slices = std::max<uint32_t>(1, q->max_peaks * nb5);
+    // every count bc or yc stays below (2dF+1)*speclen/2 + 1
+    if ((int)(nb5 * (idx->max_speclen / 2) + 1) > c->lgfact_n) return GX_ERR_INVLD_SIZE;
+    const int n_blocks = std::max(1, c->n_cu * 2);
+    st = ensure_scratch(c, n_blocks, max_slices);
+    if (st) return st;
+    SearchArgs a{};
+    a.buckets = idx->d_buckets;
+    a.n_buckets = idx->n_buckets;
+    a.qidx = q->idx;
+    a.moz = q->moz;
+    a.inten = q->inten;
+    a.prec = q->prec;
+    a.dm = q->has_dm ? q->dm : nullptr;
+    a.n_spectra = q->n_spectra;
+    a.dF = p->dF;
+    a.nbins = (int)idx->nbins;
+    a.min_shp = p->min_shp;
+    a.min_cpsm = p->min_cpsm;
+    a.dM = p->dM;
+    a.expect_max = p->expect_max;
+    a.lgfact = c->d_lgfact;
+    a.lgfact_n = c->lgfact_n;
+    a.out = q->psm;
+    a.hist_out = want_hist ? q->hist : nullptr;
+    a.work_counter = c->d_counter;
+    a.slice_scratch = c->d_slices;
+    a.cand_scratch = c->d_cand;
+    a.max_slices = max_slices;
+    a.stats = c->d_stats;
+    GX_HIP(launch_reset(c->d_counter, c->d_stats, c->stream));
+    GX_HIP(hipEventRecord(c->ev0, c->stream));
+    if (q->n_spectra > 0) GX_HIP(launch_search(a, n_blocks, c->stream));
+    GX_HIP(hipEventRecord(c->ev1, c->stream));
+    GX_HIP(hipStreamSynchronize(c->stream));
+    q->searched = true;
+    if (stats) {
+        unsigned long long h[ST_N];
+        GX_HIP(hipMemcpy(h, c->d_stats, sizeof(h), hipMemcpyDeviceToHost));
+        float ms = 0;
+        GX_HIP(hipEventElapsedTime(&ms, c->ev0, c->ev1));
+        std::memset(stats, 0, sizeof(*stats));
+        stats->ions = h[ST_IONS];
+        stats->bins = h[ST_BINS];
+        stats->window = h[ST_WINDOW];
+        stats->candidates = h[ST_CAND];
+        stats->spectra = h[ST_SPECTRA];
+        stats->peaks = h[ST_PEAKS];
+        stats->fallbacks = h[ST_FALLBACK];
+        stats->kernel_ms = ms;
+        stats->evalue_ms = 0;
+    }
+    return GX_SUCCESS;
+}
+
+extern "C" int gx_search_device(gx_ctx *c, const gx_dindex *idx, gx_dqueries *q, const gx_search_params *p,
+                                gx_search_stats *stats) {
+    if (!c || !idx || !q || !p) return GX_ERR_INVLD_PTR;
+    std::lock_guard<std::mutex> lk(c->mu);
+    return search_locked(c, idx, q, p, stats, q->hist != nullptr);
+}
+
+extern "C" int gx_results_download(gx_ctx *c, gx_dqueries *q, gx_psm *psms, uint32_t *hist) {
+    if (!c || !q) return GX_ERR_INVLD_PTR;
+    if (!q->searched) return GX_ERR_NOT_READY;
+    if (hist && !q->hist) return GX_ERR_NOT_READY;
+    std::lock_guard<std::mutex> lk(c->mu);
+    if (hipSetDevice(c->device) != hipSuccess) return GX_ERR_DEVICE;
+    if (psms && q->n_spectra)
+        GX_HIP(hipMemcpy(psms, q->psm, sizeof(gx_psm) * q->n_spectra, hipMemcpyDeviceToHost));
+    if (hist && q->n_spectra)
+        GX_HIP(hipMemcpy(hist, q->hist, sizeof(uint32_t) * GX_HIST_BINS * q->n_spectra, hipMemcpyDeviceToHost));
+    return GX_SUCCESS;
+}
+
+extern "C" int gx_search(gx_ctx *c, const gx_dindex *idx, const gx_query_batch *qb, const gx_search_params *p,
+                         gx_psm *psms, uint32_t *hist, gx_search_stats *stats) {
+    if (!c || !idx || !qb || !p || !psms) return GX_ERR_INVLD_PTR;
+    gx_dqueries *q = nullptr;
+    int st = gx_queries_upload(c, qb, &q);
+    if (st) return st;
+    {
+        std::lock_guard<std::mutex> lk(c->mu);
+        st = search_locked(c, idx, q, p, stats, hist != nullptr);
+    }
+    if (!st) st = gx_results_download(c, q, psms, hist);
+    gx_queries_free(q);
+    return st;
+}

@@ -1,0 +1,131 @@
+"""GPU search: the host side of the reference's s3::search / s4::processResults.
+
+`Searcher` owns a device context (s3::initialize, superstep3/kernel.cu:251-261)
+and a device-resident index (s1::getdIndex, superstep1/kernel.cu:310-358).
+`Searcher.search()` runs one query batch the way DSLIM_QuerySpectrum does
+(dslim_query.cpp:749-1081) and returns one record per spectrum (the top hit's
+hCell fields, cpsms and the tail-fit e-value) plus, on request, the 802-bin
+survival histograms. Every call goes through libgicops_hip.so; there is no
+CPU fallback: a missing library or device raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import _ffi as F
+from .slm import QueryBatch, SlmIndex
+
+
+@dataclass
+class SearchParams:
+    """gParams subset (slm_dsts.h:349-431) with the argp derivations
+    (argp.hpp:272-370): dF = (uint)(0.02*scale) bins, base intensities are
+    the query generator's."""
+    dF: int = 2
+    scale: int = 100
+    max_mass: int = 5000
+    min_shp: int = 4
+    min_cpsm: int = 4
+    dM: float = 10.0
+    expect_max: float = 20.0
+    evalue_mode: int = F.GX_EVALUE_TAILFIT
+    surrogate_conf: float = 0.5
+
+    def to_ffi(self) -> F.gx_search_params:
+        p = F.gx_search_params()
+        p.dF, p.scale, p.max_mass = self.dF, self.scale, self.max_mass
+        p.min_shp, p.min_cpsm, p.evalue_mode = self.min_shp, self.min_cpsm, self.evalue_mode
+        p.dM, p.expect_max, p.surrogate_conf = self.dM, self.expect_max, self.surrogate_conf
+        return p
+
+
+def stats_dict(st: F.gx_search_stats) -> dict:
+    return {k: getattr(st, k) for k, _ in F.gx_search_stats._fields_}
+
+
+class DeviceQueries:
+    """A query batch resident in HBM (dQueries, superstep3/kernel.cu:192-206)."""
+
+    def __init__(self, searcher: "Searcher", batch: QueryBatch, with_dm: bool = True):
+        self._lib = searcher._lib
+        self._s = searcher
+        self.n_spectra = batch.n_spectra
+        q = batch.as_ffi(with_dm)
+        h = C.c_void_p()
+        F.check("gx_queries_upload", self._lib.gx_queries_upload(searcher._ctx, C.byref(q), C.byref(h)))
+        self._h = h
+
+    def free(self) -> None:
+        if getattr(self, "_h", None):
+            self._lib.gx_queries_free(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+class Searcher:
+    def __init__(self, index: SlmIndex, device: int = 0):
+        self._lib = F.hip_lib()
+        ctx = C.c_void_p()
+        F.check("gx_ctx_create", self._lib.gx_ctx_create(device, C.byref(ctx)))
+        self._ctx = ctx
+        self.index = index
+        d = C.c_void_p()
+        F.check("gx_index_upload", self._lib.gx_index_upload(ctx, index.raw_array, len(index.buckets),
+                                                              index.maxz, C.byref(d)))
+        self._didx = d
+
+    def device_info(self) -> tuple[str, int]:
+        buf = C.create_string_buffer(256)
+        ncu = C.c_int()
+        F.check("gx_ctx_device_info", self._lib.gx_ctx_device_info(self._ctx, buf, 256, C.byref(ncu)))
+        return buf.value.decode(), ncu.value
+
+    def upload(self, batch: QueryBatch, with_dm: bool = True) -> DeviceQueries:
+        return DeviceQueries(self, batch, with_dm)
+
+    def search_device(self, dq: DeviceQueries, params: SearchParams) -> dict:
+        st = F.gx_search_stats()
+        p = params.to_ffi()
+        F.check("gx_search_device",
+                self._lib.gx_search_device(self._ctx, self._didx, dq._h, C.byref(p), C.byref(st)))
+        return stats_dict(st)
+
+    def download(self, dq: DeviceQueries, want_hist: bool = False):
+        psm = np.zeros(dq.n_spectra, dtype=F.PSM_DTYPE)
+        hist = np.zeros((dq.n_spectra, F.GX_HIST_BINS), dtype=np.uint32) if want_hist else None
+        F.check("gx_results_download",
+                self._lib.gx_results_download(self._ctx, dq._h, F.ptr(psm), F.ptr(hist)))
+        return psm, hist
+
+    def search(self, batch: QueryBatch, params: SearchParams, want_hist: bool = False, with_dm: bool = True):
+        """Upload, search, model and download one batch -> (psms, hist | None, stats)."""
+        psm = np.zeros(batch.n_spectra, dtype=F.PSM_DTYPE)
+        hist = np.zeros((batch.n_spectra, F.GX_HIST_BINS), dtype=np.uint32) if want_hist else None
+        st = F.gx_search_stats()
+        q = batch.as_ffi(with_dm)
+        p = params.to_ffi()
+        F.check("gx_search", self._lib.gx_search(self._ctx, self._didx, C.byref(q), C.byref(p), F.ptr(psm),
+                                                 F.ptr(hist), C.byref(st)))
+        return psm, hist, stats_dict(st)
+
+    def close(self) -> None:
+        if getattr(self, "_didx", None):
+            self._lib.gx_index_free(self._didx)
+            self._didx = None
+        if getattr(self, "_ctx", None):
+            self._lib.gx_ctx_destroy(self._ctx)
+            self._ctx = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

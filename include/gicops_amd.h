@@ -1,0 +1,191 @@
+/*
+ * gicops_amd.h - C ABI of the MI355X (gfx950) fragment-ion-index search.
+ *
+ * This is the drop-in boundary for the reference's search + e-value stage:
+ *
+ *   reference (C++/CUDA)                                this ABI
+ *   -------------------------------------------------   -----------------------------
+ *   hcp::gpu::cuda::s3::initialize()                    gx_ctx_create()
+ *     (include/cuda/superstep3/kernel.hpp:96,
+ *      superstep3/kernel.cu:251-261)
+ *   hcp::gpu::cuda::s3::deinitialize()                  gx_ctx_destroy()
+ *     (kernel.hpp:100)
+ *   hcp::gpu::cuda::s1::getdIndex(Index*)               gx_index_upload()
+ *     (superstep1/kernel.cu:310-358)
+ *   s3::search(Queries<int>*, Index*, uint, int, hCell*) gx_search()  (H2D + search + D2H)
+ *     (kernel.hpp:98) + s4::processResults(Index*,        gx_search_device() (device-resident)
+ *     Queries<int>*, int) (superstep4/kernel.hpp:49)
+ *   DSLIM_QuerySpectrum(Queries<int>*, Index*, uint, int) gx_search() with the same
+ *     (include/dslim.h:196, dslim_query.cpp:749)         per-spectrum semantics
+ *   expeRT::ModelTailFit(Results*) (expeRT.cpp:396)      in-kernel (GX_EVALUE_TAILFIT)
+ *
+ * Status codes are the reference's (source/core/include/slmerr.h:24-46) plus
+ * GX_ERR_DEVICE / GX_ERR_NOT_READY / GX_ERR_NO_DEVICE. No function exits the
+ * process or throws across the ABI: device errors come back as GX_ERR_DEVICE
+ * (the reference calls exit(), cuda/driver.hpp:41-48).
+ *
+ * Host buffers are caller-owned; device buffers are library-owned. A context
+ * serialises its own calls (internal mutex); use one context per device.
+ */
+#ifndef GICOPS_AMD_H
+#define GICOPS_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes (slmerr.h:24-46) ---------------------------------- */
+#define GX_SUCCESS 0
+#define GX_ERR_BASE (-3000)
+#define GX_ERR_INVLD_MEMORY (GX_ERR_BASE - 1)
+#define GX_ERR_INVLD_PARAM (GX_ERR_BASE - 2)
+#define GX_ERR_INVLD_INDEX (GX_ERR_BASE - 3)
+#define GX_ERR_INVLD_SIZE (GX_ERR_BASE - 4)
+#define GX_ERR_ALREADY_INIT (GX_ERR_BASE - 5)
+#define GX_ERR_FILE_NOT_FOUND (GX_ERR_BASE - 6)
+#define GX_ERR_BAD_MEM_ALLOC (GX_ERR_BASE - 7)
+#define GX_ERR_INVLD_PTR (GX_ERR_BASE - 8)
+#define GX_ERR_INVLD_SORT (GX_ERR_BASE - 9)
+#define GX_ERR_INVLD_MOD (GX_ERR_BASE - 10)
+#define GX_ERR_NOT_ENOUGH_DATA (GX_ERR_BASE - 12)
+#define GX_ERR_FILE_ERROR (GX_ERR_BASE - 13)
+/* extensions of this library */
+#define GX_ERR_DEVICE (GX_ERR_BASE - 50)    /* HIP runtime / launch failure */
+#define GX_ERR_NOT_READY (GX_ERR_BASE - 51) /* object used before it was built */
+#define GX_ERR_NO_DEVICE (GX_ERR_BASE - 52) /* no gfx950 device visible */
+
+/* ---- index: one bucket per peptide length (slm_dsts.h:298-346) ------- */
+/* Read-only view of one length bucket of the SLM index, in the reference's
+ * layout: entries sorted ascending by float precursor mass; bA[b] = number of
+ * ions in bins < b (nbins+1 words); iA = ion ids ascending within each bin;
+ * ion id = psid*speclen + r with speclen = (L-1)*maxz*2, r < speclen/2 a
+ * b-ion. The fields after `iA` are metadata the search does not read. */
+typedef struct gx_bucket {
+    int32_t peplen;        /* L */
+    uint32_t n_peps;       /* unmodified sequences (lclpepCnt) */
+    uint32_t n_entries;    /* peptides + variants (lcltotCnt) */
+    uint32_t speclen;      /* (L-1)*maxz*2 */
+    uint64_t n_ions;       /* n_entries*speclen = bA[nbins] */
+    uint32_t nbins;        /* max_mass*scale */
+    const char *seqs;      /* n_peps*L residues */
+    const float *mass;     /* n_entries, ascending */
+    const uint32_t *seqid; /* n_entries */
+    const uint32_t *modnum;/* n_entries */
+    const uint64_t *sites; /* n_entries */
+    const uint32_t *bA;    /* nbins+1 */
+    const uint32_t *iA;    /* n_ions */
+} gx_bucket;
+
+/* ---- query batch (Queries<int>, slm_dsts.h:520-658) ------------------ */
+typedef struct gx_query_batch {
+    uint32_t n_spectra;
+    uint64_t n_peaks;        /* idx[n_spectra] */
+    const uint32_t *idx;     /* n_spectra+1 CSR row offsets */
+    const int32_t *moz;      /* n_peaks, (int)(m/z*scale) */
+    const int32_t *intensity;/* n_peaks */
+    const float *precurse;   /* n_spectra, precursor mass */
+    const float *dm;         /* optional n_spectra per-spectrum tolerance (Da);
+                                NULL = params.dM for all (an extension: the
+                                reference has one global dM) */
+} gx_query_batch;
+
+/* ---- parameters (gParams subset, slm_dsts.h:349-431, argp.hpp:272-370) */
+enum { GX_EVALUE_TAILFIT = 0, GX_EVALUE_LEARNED = 1 };
+
+typedef struct gx_search_params {
+    int32_t dF;            /* fragment tolerance in bins ((uint)(0.02*scale) = 2) */
+    int32_t scale;         /* bins per Da (100) */
+    int32_t max_mass;      /* Da (5000); legal peaks dF < q < max_mass*scale-1-dF */
+    int32_t min_shp;       /* min shared peaks (4) */
+    int32_t min_cpsm;      /* min candidate PSMs for an e-value (4) */
+    int32_t evalue_mode;   /* GX_EVALUE_* */
+    double dM;             /* precursor tolerance, Da (global) */
+    double expect_max;     /* report gate: e < expect_max (20) */
+    float surrogate_conf;  /* GX_EVALUE_LEARNED: confidence threshold for the MLP */
+    int32_t reserved;
+} gx_search_params;
+
+/* ---- per-spectrum result (hCell + Results + e-value) ----------------- */
+enum {
+    GX_PSM_HAS_HIT = 1,    /* cpsms >= 1: a top hit exists */
+    GX_PSM_MODELED = 2,    /* cpsms >= min_cpsm: the tail fit ran */
+    GX_PSM_FIT_OK = 4,     /* the fit is defined (see the flat-slice rule) */
+    GX_PSM_REPORTED = 8,   /* e < expect_max: DFile_PrintScore would print it */
+    GX_PSM_SURROGATE = 16, /* e-value came from the learned surrogate */
+    GX_PSM_TIE = 32        /* another candidate had the same top hyperscore */
+};
+
+typedef struct gx_psm {
+    float hyperscore;    /* top hit hyperscore (f32, clamped to 79), 0 if none */
+    int32_t psid;        /* index into the bucket's mass-sorted entries, -1 if none */
+    uint16_t idxoffset;  /* bucket (length) index of the top hit */
+    uint16_t sharedions; /* bc+yc of the top hit */
+    uint16_t totalions;  /* speclen of the top hit's bucket */
+    uint16_t flags;      /* GX_PSM_* */
+    int32_t cpsms;       /* candidate PSMs (h > 0, shared >= min_shp) */
+    int32_t hyp;         /* (int)(hyperscore*10 + 0.5): histogram bin of the top hit */
+    int32_t mu;          /* tail-fit slope * 1e6, truncated (Results::mu) */
+    int32_t beta;        /* tail-fit intercept * 1e6, truncated (Results::beta) */
+    double evalue;       /* cpsms * 10^(mu/1e6*hyp + beta/1e6); +inf if not modeled */
+} gx_psm;
+
+#define GX_HIST_BINS 802 /* 2 + MAX_HYPERSCORE*10 (kernel.hpp:32) */
+
+/* Work counters of one search call, summed over spectra (window-restricted
+ * ion reads, bA pairs, window entries, candidates, tail-fit fallbacks). */
+typedef struct gx_search_stats {
+    uint64_t ions;        /* H: iA entries inside the precursor windows */
+    uint64_t bins;        /* B: (bucket, peak, bin) triples whose bA pair was read */
+    uint64_t window;      /* W: index entries inside the precursor windows */
+    uint64_t candidates;  /* sum of cpsms */
+    uint64_t spectra;     /* spectra searched */
+    uint64_t peaks;       /* query peaks read */
+    uint64_t fallbacks;   /* learned mode: spectra routed to the plain tail fit */
+    double kernel_ms;     /* device time of the search kernels (HIP events) */
+    double evalue_ms;     /* device time of the e-value stage (0 if fused) */
+} gx_search_stats;
+
+typedef struct gx_ctx gx_ctx;
+typedef struct gx_dindex gx_dindex;
+typedef struct gx_dqueries gx_dqueries;
+
+/* Create a context on HIP device `device` (lgfact table, stream, events).
+ * Replaces s3::initialize(). */
+int gx_ctx_create(int device, gx_ctx **out);
+void gx_ctx_destroy(gx_ctx *ctx);
+
+/* Upload an index (all length buckets) to HBM; replaces s1::getdIndex().
+ * `maxz` is the fragment charge count the index was built with. Buckets with
+ * n_entries == 0 are kept but never searched. */
+int gx_index_upload(gx_ctx *ctx, const gx_bucket *buckets, int n_buckets, int maxz, gx_dindex **out);
+void gx_index_free(gx_dindex *idx);
+
+/* Upload a query batch to HBM (dQueries::H2D, superstep3/kernel.cu:192-206). */
+int gx_queries_upload(gx_ctx *ctx, const gx_query_batch *q, gx_dqueries **out);
+void gx_queries_free(gx_dqueries *q);
+
+/* Search a device-resident batch and model its e-values; results stay on the
+ * device until gx_results_download(). Replaces s3::search +
+ * s4::processResults for one batch. */
+int gx_search_device(gx_ctx *ctx, const gx_dindex *idx, gx_dqueries *q, const gx_search_params *p,
+                     gx_search_stats *stats);
+
+/* Copy the results of the last gx_search_device() on `q` to the host.
+ * `hist` (optional) receives n_spectra*GX_HIST_BINS u32 survival counts. */
+int gx_results_download(gx_ctx *ctx, gx_dqueries *q, gx_psm *psms, uint32_t *hist);
+
+/* One call: upload `q`, search, model, download (s3::search semantics). */
+int gx_search(gx_ctx *ctx, const gx_dindex *idx, const gx_query_batch *q, const gx_search_params *p,
+              gx_psm *psms, uint32_t *hist, gx_search_stats *stats);
+
+/* Name of the device the context runs on and its CU count (diagnostics). */
+int gx_ctx_device_info(gx_ctx *ctx, char *name, size_t name_len, int *n_cu);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* GICOPS_AMD_H */

@@ -1,0 +1,146 @@
+"""HIP search vs the CPU oracle on the same seeded inputs (through the C ABI).
+
+Bar: shared-peak counts, candidate counts, histograms and top-hit ids are
+bit-exact; hyperscores within 1e-5 relative; e-values within 1e-3 relative of
+log10.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+from gicops_amd import _ffi as F
+from gicops_amd import slm
+from gicops_amd.search import SearchParams
+from oracle import pyoracle as O
+
+pytestmark = pytest.mark.gpu
+
+HYP_RTOL = 1e-5
+LOGE_RTOL = 1e-3
+EXACT = ["psid", "idxoffset", "sharedions", "totalions", "cpsms", "hyp", "flags", "mu", "beta"]
+
+
+def say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+def compare(gpu, cpu, hg=None, hc=None, label=""):
+    assert len(gpu) == len(cpu)
+    for f in EXACT:
+        bad = np.nonzero(gpu[f] != cpu[f])[0]
+        assert len(bad) == 0, f"{label}: field {f} differs at {bad[:10]}: gpu={gpu[f][bad[:5]]} cpu={cpu[f][bad[:5]]}"
+    h_g, h_c = gpu["hyperscore"].astype(np.float64), cpu["hyperscore"].astype(np.float64)
+    rel = np.abs(h_g - h_c) / np.maximum(np.abs(h_c), 1e-30)
+    assert np.all(rel <= HYP_RTOL), f"{label}: hyperscore rel err {rel.max()}"
+    fin = np.isfinite(cpu["evalue"])
+    assert np.array_equal(fin, np.isfinite(gpu["evalue"])), label
+    lg, lc = np.log10(gpu["evalue"][fin]), np.log10(cpu["evalue"][fin])
+    err = np.abs(lg - lc) / np.maximum(np.abs(lc), 1e-12)
+    assert np.all((err <= LOGE_RTOL) | (np.abs(lg - lc) < 1e-9)), f"{label}: log10 e rel err {err.max()}"
+    if hg is not None:
+        assert np.array_equal(hg, hc), f"{label}: histograms differ"
+
+
+def check_stats(st, cnt, n):
+    assert st["spectra"] == n
+    assert st["ions"] == int(cnt["ions"].sum())
+    assert st["bins"] == int(cnt["bins"].sum())
+    assert st["window"] == int(cnt["window"].sum())
+
+
+def test_small_closed_search_parity(small_index, small_batch, searcher_small):
+    say("case: small closed search, 50k peptides, 1000 spectra, dM 10")
+    p = SearchParams(dM=10.0)
+    g, hg, st = searcher_small.search(small_batch, p, want_hist=True)
+    c, hc, cnt = O.search(small_index, small_batch, O.default_params(dM=10.0), want_hist=True,
+                          want_counters=True)
+    compare(g, c, hg, hc, "small")
+    check_stats(st, cnt, small_batch.n_spectra)
+    assert st["candidates"] == int(c["cpsms"].sum())
+
+
+def test_open_search_parity(open_index, searcher_open):
+    say("case: open search, 200k peptides + Met-ox, 300 spectra, dM 500")
+    cfg = slm.SpecConfig(n_spectra=300, seed=21, peak_mode=F.GX_PEAKS_TOTAL_UNIFORM, peaks_min=50,
+                         peaks_max=150, charge_min=2, charge_max=3, dm_choices=(500.0,))
+    batch = slm.generate_spectra(open_index, cfg)
+    g, hg, st = searcher_open.search(batch, SearchParams(dM=500.0), want_hist=True)
+    c, hc, cnt = O.search(open_index, batch, O.default_params(dM=500.0), want_hist=True, want_counters=True)
+    compare(g, c, hg, hc, "open")
+    check_stats(st, cnt, batch.n_spectra)
+    assert (c["flags"] & F.GX_PSM_MODELED).sum() > 250
+
+
+def test_uneven_batch_parity(open_index, searcher_open):
+    """power-law peak counts up to 2000 (slice tables spill to scratch) and
+    per-spectrum dM from {10, 100, 500}"""
+    say("case: uneven batch, power-law peaks 20-2000, dM mix")
+    cfg = slm.SpecConfig(n_spectra=160, seed=33, peak_mode=F.GX_PEAKS_TOTAL_POWER, peaks_min=20,
+                         peaks_max=2000, power_alpha=1.2, charge_min=2, charge_max=3, qalen=2048,
+                         dm_choices=(10.0, 100.0, 500.0))
+    batch = slm.generate_spectra(open_index, cfg)
+    assert (np.diff(batch.idx) > 1024 // 5).any()
+    g, hg, st = searcher_open.search(batch, SearchParams(dM=500.0), want_hist=True)
+    c, hc, cnt = O.search(open_index, batch, O.default_params(dM=500.0), want_hist=True, want_counters=True)
+    compare(g, c, hg, hc, "uneven")
+    check_stats(st, cnt, batch.n_spectra)
+
+
+def test_device_resident_path_matches_one_call(small_batch, searcher_small):
+    say("case: device-resident upload/search/download vs one-call search")
+    p = SearchParams(dM=10.0)
+    g1, _, _ = searcher_small.search(small_batch, p)
+    dq = searcher_small.upload(small_batch)
+    searcher_small.search_device(dq, p)
+    searcher_small.search_device(dq, p)  # repeatable: state is reset per call
+    g2, _ = searcher_small.download(dq)
+    dq.free()
+    assert g1.tobytes() == g2.tobytes()
+
+
+def _edge_batch(index):
+    b = index.buckets[10]
+    m = float(b.mass[len(b.mass) // 2])
+    base = slm.generate_spectra(index, slm.SpecConfig(n_spectra=4, seed=99))
+    specs = [base.peaks(i) for i in range(4)]
+    empty = (np.zeros(0, np.int32), np.zeros(0, np.int32))
+    illegal = (np.array([0, 1, 2, 499998, 499999, -5], np.int32), np.full(6, 1000, np.int32))
+    dup = (np.repeat(specs[0][0], 3), np.repeat(specs[0][1], 3))  # every peak three times
+    rows = [specs[0], empty, illegal, specs[1], dup, specs[2], specs[3]]
+    prec = [base.precurse[0], m, m, 1.0e6, base.precurse[0], -50.0, base.precurse[3]]
+    idx = np.zeros(len(rows) + 1, np.uint32)
+    idx[1:] = np.cumsum([len(r[0]) for r in rows])
+    return slm.QueryBatch(idx, np.concatenate([r[0] for r in rows]), np.concatenate([r[1] for r in rows]),
+                          np.array(prec, np.float32))
+
+
+@pytest.mark.parametrize("kw", [dict(), dict(min_shp=1), dict(min_shp=0, min_cpsm=0), dict(dM=-1.0),
+                                dict(dM=0.0), dict(min_cpsm=1, expect_max=1e30)])
+def test_edge_cases_parity(small_index, searcher_small, kw):
+    say(f"case: edge batch {kw}")
+    batch = _edge_batch(small_index)
+    pd = dict(dM=10.0)
+    pd.update(kw)
+    g, hg, _ = searcher_small.search(batch, SearchParams(**pd), want_hist=True)
+    c, hc, _ = O.search(small_index, batch, O.default_params(**pd), want_hist=True)
+    compare(g, c, hg, hc, f"edge {kw}")
+
+
+def test_empty_batch(searcher_small):
+    say("case: empty batch")
+    batch = slm.QueryBatch(np.zeros(1, np.uint32), np.zeros(0, np.int32), np.zeros(0, np.int32),
+                           np.zeros(0, np.float32))
+    g, _, st = searcher_small.search(batch, SearchParams())
+    assert len(g) == 0 and st["spectra"] == 0
+
+
+def test_invalid_arguments_return_status(searcher_small, small_batch):
+    say("case: invalid arguments")
+    with pytest.raises(F.GxError) as e:
+        searcher_small.search(small_batch, SearchParams(scale=50))  # nbins mismatch with the index
+    assert e.value.code == F.GX_ERR_INVLD_INDEX
+    with pytest.raises(F.GxError) as e:
+        searcher_small.search(small_batch, SearchParams(dF=-1))
+    assert e.value.code == F.GX_ERR_INVLD_PARAM
