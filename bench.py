@@ -90,6 +90,9 @@ def main() -> int:
     ap.add_argument("--spectra", type=int, default=0, help="override the batch size")
     ap.add_argument("--cpu-budget", type=float, default=15.0, help="seconds of CPU baseline work")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--engine", default="batched", choices=["batched", "spectrum"])
+    ap.add_argument("--config", type=int, default=-1, help="batched launch configuration (-1: try all, keep the best)")
+    ap.add_argument("--compare-plain", action="store_true", help="also time the per-spectrum kernel once")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for {args.gpus}); the search shards by "
@@ -116,10 +119,30 @@ def main() -> int:
     s = Searcher(index)
     name, ncu = s.device_info()
     log(f"device: {name}, {ncu} CUs")
-    params = SearchParams(dM=dM)
+    from gicops_amd import _ffi as F
+    engine = F.GX_ENGINE_BATCHED if args.engine == "batched" else F.GX_ENGINE_SPECTRUM
+    params = SearchParams(dM=dM, engine=engine, config=args.config)
     t = time.perf_counter()
     dq = s.upload(batch)
     log(f"query upload {time.perf_counter() - t:.2f} s")
+    extra = {}
+    if engine == F.GX_ENGINE_BATCHED and args.config < 0:
+        best = None
+        for cfg in range(3):
+            params.config = cfg
+            st = s.search_device(dq, params)
+            log(f"config {cfg}: total {st['total_ms']:.1f} ms (plan {st['plan_ms']:.1f}, search {st['kernel_ms']:.1f}, "
+                f"evalue {st['evalue_ms']:.1f}), {st['items']} items, {st['chunks']} chunks")
+            extra[f"config{cfg}_total_ms"] = round(st["total_ms"], 3)
+            if best is None or st["total_ms"] < best[1]:
+                best = (cfg, st["total_ms"])
+        params.config = best[0]
+    if args.compare_plain:
+        pp = SearchParams(dM=dM, engine=F.GX_ENGINE_SPECTRUM)
+        st = s.search_device(dq, pp)
+        extra["per_spectrum_kernel_ms"] = round(st["kernel_ms"], 3)
+        extra["per_spectrum_spectra_per_s"] = round(batch.n_spectra / (st["kernel_ms"] / 1000.0), 1)
+        log(f"per-spectrum kernel: {st['kernel_ms']:.1f} ms")
     for i in range(args.warmup):
         st = s.search_device(dq, params)
         log(f"warmup {i}: kernel {st['kernel_ms']:.1f} ms")
@@ -128,7 +151,7 @@ def main() -> int:
     for i in range(args.steps):
         st = s.search_device(dq, params)
         kms.append(st["kernel_ms"])
-        log(f"step {i}: kernel {st['kernel_ms']:.1f} ms")
+        log(f"step {i}: total {st['total_ms']:.1f} ms, search kernel {st['kernel_ms']:.1f} ms")
     t1 = time.perf_counter()
     psm, _ = s.download(dq)
     ms_step = (t1 - t0) * 1000.0 / args.steps
@@ -169,7 +192,9 @@ def main() -> int:
             "dM_da": dM,
             "dF_bins": params.dF,
             "evalue": "plain tail fit",
-            "launch": "persistent work-stealing, 2 blocks/CU",
+            "engine": args.engine,
+            "launch_config": params.config,
+            **extra,
         },
         "roofline": {
             "bound": "hbm",
@@ -179,7 +204,7 @@ def main() -> int:
             "frac": round(achieved / HBM_PEAK_GBS, 4),
             "traffic": traffic,
             "algorithmic_bytes_per_launch": abytes,
-            "kernel": "search_kernel",
+            "kernel": "search_items" if engine == F.GX_ENGINE_BATCHED else "search_kernel",
             "kernel_ms": round(avg_kms, 3),
         },
         "cpu_baseline": cpu,

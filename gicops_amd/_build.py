@@ -32,6 +32,7 @@ OFFLOAD_ARCH = "gfx950"
 HOST_SOURCES = [CSRC / "host" / "slm_build.cpp"]
 HIP_SOURCES = [
     CSRC / "hip" / "search.hip",
+    CSRC / "hip" / "batched.hip",
     CSRC / "hip" / "capi.hip",
 ]
 HIP_HEADERS = sorted((CSRC / "hip").glob("*.h")) + sorted(INCLUDE.glob("*.h"))

@@ -75,7 +75,8 @@ class gx_search_params(C.Structure):
     _fields_ = [
         ("dF", C.c_int32), ("scale", C.c_int32), ("max_mass", C.c_int32), ("min_shp", C.c_int32),
         ("min_cpsm", C.c_int32), ("evalue_mode", C.c_int32), ("dM", C.c_double),
-        ("expect_max", C.c_double), ("surrogate_conf", C.c_float), ("reserved", C.c_int32),
+        ("expect_max", C.c_double), ("surrogate_conf", C.c_float), ("engine", C.c_int32),
+        ("config", C.c_int32), ("use_graph", C.c_int32),
     ]
 
 
@@ -83,7 +84,8 @@ class gx_search_stats(C.Structure):
     _fields_ = [
         ("ions", C.c_uint64), ("bins", C.c_uint64), ("window", C.c_uint64), ("candidates", C.c_uint64),
         ("spectra", C.c_uint64), ("peaks", C.c_uint64), ("fallbacks", C.c_uint64),
-        ("kernel_ms", C.c_double), ("evalue_ms", C.c_double),
+        ("items", C.c_uint64), ("chunks", C.c_uint32), ("config", C.c_int32),
+        ("kernel_ms", C.c_double), ("evalue_ms", C.c_double), ("plan_ms", C.c_double), ("total_ms", C.c_double),
     ]
 
 
@@ -133,6 +135,7 @@ assert PSM_DTYPE.itemsize == 40
 GX_HIST_BINS = 802
 GX_PSM_HAS_HIT, GX_PSM_MODELED, GX_PSM_FIT_OK, GX_PSM_REPORTED, GX_PSM_SURROGATE, GX_PSM_TIE = 1, 2, 4, 8, 16, 32
 GX_EVALUE_TAILFIT, GX_EVALUE_LEARNED = 0, 1
+GX_ENGINE_BATCHED, GX_ENGINE_SPECTRUM = 0, 1
 GX_PEAKS_NOISE_UNIFORM, GX_PEAKS_TOTAL_UNIFORM, GX_PEAKS_TOTAL_POWER = 0, 1, 2
 
 

@@ -1,0 +1,114 @@
+// Batched engine: plan kernels, the persistent work-stealing search over
+// (spectrum, bucket, tile) items, and the e-value kernel.
+#pragma once
+
+#include "search_common.h"
+
+namespace gx {
+
+// One search configuration (the autotuner's choice): block size and the width
+// of the LDS count tile. Counts are packed 4 (u8) or 2 (u16) per 32-bit word.
+struct BatchConfig {
+    int threads;     // 256, 512 or 1024
+    int tile_words;  // count-tile words per block
+    int slice_chunk; // slices staged in LDS at a time
+    int blocks_per_cu;
+};
+
+constexpr int kNumBatchConfigs = 3;
+BatchConfig batch_config(int id);
+
+struct PlanArgs {
+    const DevBucket *buckets;
+    int n_buckets;
+    const uint32_t *qidx;
+    const int32_t *moz;
+    const int32_t *inten;
+    const float *prec;
+    const float *dm;
+    uint32_t n_spectra;
+    int dF;
+    int nbins;
+    double dM;
+    int tile_words;
+    // outputs of plan_spectra
+    int32_t *lq;        // sorted legal peaks (CSR at qidx offsets)
+    int32_t *li;
+    uint32_t *n_legal;  // per spectrum
+    int32_t *pair_lo;   // [s*nb + b]
+    int32_t *pair_hi;
+    uint32_t *pair_tiles;    // tiles of the pair (0 = no window)
+    uint32_t *pair_pack;     // entries per count word (4 or 2)
+    uint32_t *pair_item_off; // spectrum-local item offset of the pair
+    uint32_t *pair_pos_off;  // spectrum-local slice-position offset (words)
+    uint32_t *n_items;       // per spectrum, scanned in place to offsets by scan_kernel
+    uint64_t *n_pos;         // per spectrum, scanned in place to offsets
+    // scanned offsets (n_spectra + 1 entries)
+    uint32_t *item_off;
+    uint64_t *pos_off;
+    // outputs of plan_slices
+    uint32_t *pos;           // chunk-local slice positions: [a, z) per slice
+    uint32_t *items;         // 2 words per item: s, (b | t << 8)
+    uint32_t chunk_s0;       // first spectrum of the current chunk
+    uint32_t chunk_s1;
+    unsigned long long *stats;
+};
+
+struct ItemArgs {
+    const DevBucket *buckets;
+    int n_buckets;
+    const uint32_t *qidx;
+    int dF;
+    int min_shp;
+    const double *lgfact;
+    int lgfact_n;
+    const int32_t *lq;
+    const int32_t *li;
+    const uint32_t *n_legal;
+    const int32_t *pair_lo;
+    const int32_t *pair_hi;
+    const uint32_t *pair_tiles;
+    const uint32_t *pair_pack;
+    const uint32_t *pair_pos_off;
+    const uint64_t *pos_off;
+    const uint32_t *pos;
+    const uint32_t *items;
+    uint32_t item0, item1;   // item range of the chunk
+    uint32_t chunk_s0;
+    uint32_t *work_counter;  // this chunk's counter
+    uint32_t *cand_scratch;  // per block overflow candidates
+    int tile_entries_max;
+    // per-spectrum accumulators
+    uint32_t *hist;          // n_spectra * GX_HIST_BINS
+    uint32_t *cpsms;
+    // per-item top hit: key (h bits << 32 | ~(b << 26 | psid)), equal count, shared ions
+    unsigned long long *item_key;
+    uint32_t *item_eq;
+    uint32_t *item_shp;
+    unsigned long long *stats;
+};
+
+struct EvalArgs {
+    const DevBucket *buckets;
+    uint32_t n_spectra;
+    int min_cpsm;
+    double expect_max;
+    const uint32_t *item_off;
+    const unsigned long long *item_key;
+    const uint32_t *item_eq;
+    const uint32_t *item_shp;
+    const uint32_t *hist;
+    const uint32_t *cpsms;
+    const uint8_t *route;    // learned mode: 1 = surrogate result already written; null = fit all
+    gx_psm *out;
+    unsigned long long *stats;
+};
+
+hipError_t launch_fill_u32(uint32_t *p, uint64_t n, uint32_t v, hipStream_t s);
+hipError_t launch_plan_spectra(const PlanArgs &a, hipStream_t s);
+hipError_t launch_scan(const PlanArgs &a, hipStream_t s);
+hipError_t launch_plan_slices(const PlanArgs &a, hipStream_t s);
+hipError_t launch_search_items(int config, const ItemArgs &a, int n_blocks, hipStream_t s);
+hipError_t launch_evalue(const EvalArgs &a, hipStream_t s);
+
+}  // namespace gx

@@ -1,0 +1,637 @@
+// Batched fragment-ion search for gfx950: plan kernels + one persistent,
+// work-stealing search launch over (spectrum, bucket, tile) items + the
+// e-value kernel. Same arithmetic as DSLIM_QuerySpectrum
+// (dslim_query.cpp:749-1081) and ModelTailFit (expeRT.cpp:396-559); the
+// order of work is what changes:
+//
+//   plan_spectra  one block per spectrum: legal peaks compacted and sorted by
+//                 m/z, precursor windows of every bucket, the per-spectrum
+//                 bound on shared-peak counts (which picks 8-, 16- or 32-bit
+//                 LDS counters) and the number of psid tiles per window;
+//   scan          item and slice-position offsets per spectrum;
+//   plan_slices   one block per (spectrum, bucket): every (peak, bin) slice
+//                 of iA is cut to the window with two lower_bounds, run by
+//                 thousands of independent blocks so their latency overlaps;
+//   search_items  persistent blocks pull items from a global counter; an item
+//                 streams its slices' ion ids in 64-id pieces (several loads
+//                 in flight per wave), counts shared peaks with LDS atomics,
+//                 rescans candidates against the sorted peaks for exact
+//                 bc/yc/ibc/iyc, and adds its histogram, candidate count and
+//                 best hit to per-spectrum accumulators;
+//   evalue        one wave per spectrum: top hit, tail fit, e-value.
+#include "batched.h"
+#include "device_util.h"
+
+namespace gx {
+namespace {
+
+constexpr int kPlanThreads = 256;
+constexpr int kPeakCap = kMaxPeaksLds;  // 2048 legal peaks per spectrum
+constexpr int kStatSpread = kStatRows;
+constexpr int kUnroll = 4;              // pieces in flight per wave
+
+__device__ __forceinline__ void stat_add(unsigned long long *stats, int which, unsigned long long v) {
+    if (v) atomicAdd(&stats[(blockIdx.x % kStatSpread) * ST_N + which], v);
+}
+
+// exact floor(raw / d) for raw < 2^32 and 1 <= d < 2^12: (raw + 0.5)/d is at
+// least 0.5/d away from an integer, far more than the double rounding error
+__device__ __forceinline__ uint32_t div_entry(uint32_t raw, double inv_d) {
+    return (uint32_t)(((double)raw + 0.5) * inv_d);
+}
+
+__global__ void fill_u32_kernel(uint32_t *p, uint64_t n, uint32_t v) {
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+        p[i] = v;
+}
+
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(kPlanThreads) plan_spectra_kernel(PlanArgs a) {
+    __shared__ int32_t tq[kPeakCap], ti[kPeakCap], sq[kPeakCap];
+    __shared__ uint32_t s_nlegal;
+    __shared__ int s_maxwin;
+    __shared__ uint32_t s_tiles[kMaxBuckets];
+    __shared__ unsigned long long s_win;
+    const int tid = threadIdx.x, lane = tid & 63;
+    const uint32_t s = blockIdx.x;
+    const int dF = a.dF;
+    const uint32_t off = a.qidx[s];
+    const int npk = (int)(a.qidx[s + 1] - off);
+    if (tid == 0) {
+        s_nlegal = 0;
+        s_maxwin = 0;
+        s_win = 0;
+    }
+    __syncthreads();
+    for (int base = 0; base < npk; base += kPlanThreads) {
+        const int k = base + tid;
+        int32_t q = 0, in = 0;
+        bool legal = false;
+        if (k < npk) {
+            q = a.moz[off + k];
+            in = a.inten[off + k];
+            legal = q > dF && (int64_t)q < (int64_t)a.nbins - 1 - dF;
+        }
+        const uint64_t m = __ballot(legal);
+        uint32_t wbase = 0;
+        if (lane == 0 && m) wbase = atomicAdd(&s_nlegal, (uint32_t)__popcll(m));
+        wbase = __shfl(wbase, 0, 64);
+        if (legal) {
+            const uint32_t pos = wbase + (uint32_t)__popcll(m & ((1ull << lane) - 1));
+            tq[pos] = q;
+            ti[pos] = in;
+        }
+    }
+    __syncthreads();
+    const int P = (int)s_nlegal;
+    for (int k = tid; k < P; k += kPlanThreads) {
+        const int32_t q = tq[k];
+        int r = 0;
+        for (int j = 0; j < P; j++) {
+            const int32_t o = tq[j];
+            r += (o < q) || (o == q && j < k);
+        }
+        sq[r] = q;
+        a.lq[off + r] = q;
+        a.li[off + r] = ti[k];
+    }
+    __syncthreads();
+    // most peaks any 2dF+1 consecutive bins hold: each fragment ion is counted
+    // at most this many times per entry
+    int mw = 0;
+    for (int k = tid; k < P; k += kPlanThreads) mw = max(mw, upper_bound_lds(sq, P, sq[k] + 2 * dF) - k);
+    mw = wave_max_i32(mw);
+    if (lane == 0) atomicMax(&s_maxwin, mw);
+    __syncthreads();
+    const int maxwin = s_maxwin;
+    const float prec = a.prec[s];
+    const double dM = a.dm ? (double)a.dm[s] : a.dM;
+    for (int b = tid; b < a.n_buckets; b += kPlanThreads) {
+        const DevBucket bk = a.buckets[b];
+        int32_t lo, hi;
+        const bool ok = precursor_window(bk, prec, dM, lo, hi);
+        const uint64_t bound = (uint64_t)bk.speclen * (uint64_t)maxwin;
+        const uint32_t pack = bound <= 255 ? 4u : (bound <= 65535 ? 2u : 1u);
+        const uint64_t T = (uint64_t)pack * (uint64_t)a.tile_words;
+        uint32_t tiles = 0;
+        if (ok) {
+            atomicAdd(&s_win, (unsigned long long)(hi - lo + 1));
+            if (P > 0) tiles = (uint32_t)(((uint64_t)(hi - lo + 1) + T - 1) / T);
+        }
+        const uint32_t pi = s * (uint32_t)a.n_buckets + b;
+        a.pair_lo[pi] = ok ? lo : 0;
+        a.pair_hi[pi] = ok ? hi : -1;
+        a.pair_tiles[pi] = tiles;
+        a.pair_pack[pi] = pack;
+        s_tiles[b] = tiles;
+    }
+    __syncthreads();
+    if (tid == 0) {
+        uint32_t it = 0;
+        uint64_t pw = 0;
+        const uint64_t nsl = (uint64_t)P * (uint64_t)(2 * dF + 1);
+        for (int b = 0; b < a.n_buckets; b++) {
+            const uint32_t pi = s * (uint32_t)a.n_buckets + b;
+            a.pair_item_off[pi] = it;
+            a.pair_pos_off[pi] = (uint32_t)pw;
+            it += s_tiles[b];
+            if (s_tiles[b]) pw += 2 * nsl;
+        }
+        a.n_legal[s] = (uint32_t)P;
+        a.n_items[s] = it;
+        a.n_pos[s] = pw;
+        stat_add(a.stats, ST_WINDOW, s_win);
+        stat_add(a.stats, ST_PEAKS, (unsigned long long)P);
+        stat_add(a.stats, ST_SPECTRA, 1ull);
+    }
+}
+
+// exclusive scans of the per-spectrum item and slice-position counts
+__global__ void __launch_bounds__(1024) scan_kernel(PlanArgs a) {
+    __shared__ uint32_t wsum_i[16];
+    __shared__ unsigned long long wsum_p[16];
+    __shared__ uint32_t carry_i;
+    __shared__ unsigned long long carry_p;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    if (tid == 0) {
+        carry_i = 0;
+        carry_p = 0;
+    }
+    __syncthreads();
+    for (uint32_t base = 0; base < a.n_spectra; base += 1024) {
+        const uint32_t i = base + tid;
+        const uint32_t vi = i < a.n_spectra ? a.n_items[i] : 0u;
+        const unsigned long long vp = i < a.n_spectra ? a.n_pos[i] : 0ull;
+        // inclusive wave scans
+        uint32_t si = vi;
+        unsigned long long sp = vp;
+#pragma unroll
+        for (int o = 1; o < 64; o <<= 1) {
+            const uint32_t ti = __shfl_up(si, o, 64);
+            const unsigned long long tp = __shfl_up(sp, o, 64);
+            if (lane >= o) {
+                si += ti;
+                sp += tp;
+            }
+        }
+        if (lane == 63) {
+            wsum_i[wave] = si;
+            wsum_p[wave] = sp;
+        }
+        __syncthreads();
+        uint32_t wi = 0;
+        unsigned long long wp = 0;
+        for (int w = 0; w < wave; w++) {
+            wi += wsum_i[w];
+            wp += wsum_p[w];
+        }
+        if (i < a.n_spectra) {
+            a.item_off[i] = carry_i + wi + si - vi;
+            a.pos_off[i] = carry_p + wp + sp - vp;
+        }
+        __syncthreads();
+        if (tid == 1023) {
+            carry_i += wi + si;
+            carry_p += wp + sp;
+        }
+        __syncthreads();
+    }
+    if (tid == 0) {
+        a.item_off[a.n_spectra] = carry_i;
+        a.pos_off[a.n_spectra] = carry_p;
+    }
+}
+
+// one block per (spectrum, bucket) of the chunk
+__global__ void __launch_bounds__(kPlanThreads) plan_slices_kernel(PlanArgs a) {
+    __shared__ unsigned long long s_st[2];
+    const int tid = threadIdx.x, lane = tid & 63;
+    const uint32_t nb = (uint32_t)a.n_buckets;
+    const uint32_t s = a.chunk_s0 + blockIdx.x / nb;
+    const uint32_t b = blockIdx.x % nb;
+    const uint32_t pi = s * nb + b;
+    const uint32_t tiles = a.pair_tiles[pi];
+    if (tiles == 0) return;
+    if (tid < 2) s_st[tid] = 0;
+    __syncthreads();
+    const int dF = a.dF, nb5 = 2 * dF + 1;
+    const int P = (int)a.n_legal[s];
+    const int nsl = P * nb5;
+    const DevBucket bk = a.buckets[b];
+    const uint32_t speclen = bk.speclen;
+    const uint32_t lo_id = (uint32_t)a.pair_lo[pi] * speclen;
+    const uint32_t end_id = (uint32_t)(a.pair_hi[pi] + 1) * speclen;
+    const uint64_t base = (a.pos_off[s] - a.pos_off[a.chunk_s0]) + a.pair_pos_off[pi];
+    const int32_t *lq = a.lq + a.qidx[s];
+    unsigned long long ions = 0, bins = 0;
+    for (int j = tid; j < nsl; j += kPlanThreads) {
+        const int k = j / nb5;
+        const uint32_t bin = (uint32_t)(lq[k] - dF + (j - k * nb5));
+        const uint32_t st = bk.bA[bin], en = bk.bA[bin + 1];
+        uint32_t lo = en, hi = en;
+        if (en > st) {
+            lo = st + lower_bound_u32(bk.iA + st, en - st, lo_id);
+            hi = lo + lower_bound_u32(bk.iA + lo, en - lo, end_id);
+        }
+        a.pos[base + 2 * (uint64_t)j] = lo;
+        a.pos[base + 2 * (uint64_t)j + 1] = hi;
+        ions += hi - lo;
+        bins++;
+    }
+    const uint32_t it0 = a.item_off[s] + a.pair_item_off[pi];
+    for (uint32_t t = tid; t < tiles; t += kPlanThreads) {
+        a.items[2 * (uint64_t)(it0 + t)] = s;
+        a.items[2 * (uint64_t)(it0 + t) + 1] = b | (t << 8);
+    }
+    ions = wave_sum_u64(ions);
+    bins = wave_sum_u64(bins);
+    if (lane == 0) {
+        atomicAdd(&s_st[0], ions);
+        atomicAdd(&s_st[1], bins);
+    }
+    __syncthreads();
+    if (tid == 0) {
+        stat_add(a.stats, ST_IONS, s_st[0]);
+        stat_add(a.stats, ST_BINS, s_st[1]);
+    }
+}
+
+// ---------------------------------------------------------------------------
+template <int THREADS, int TILE_WORDS, int SL>
+struct ItemLds {
+    uint32_t cnt[TILE_WORDS];
+    int32_t pk_q[kPeakCap];
+    int32_t pk_i[kPeakCap];
+    uint32_t s_a[SL];
+    uint32_t s_n[SL];
+    uint32_t s_off[SL + 1];
+    uint32_t cand[kCandLds];
+    uint32_t hist[GX_HIST_BINS];
+    uint32_t wsum[THREADS / 64];
+    unsigned long long w_key[THREADS / 64];
+    uint32_t w_eq[THREADS / 64];
+    uint32_t w_shp[THREADS / 64];
+    uint32_t w_cps[THREADS / 64];
+    uint32_t item;
+    uint32_t n_cand;
+};
+
+// exclusive block scan of v[0..n) in place (n <= SL); writes the total to v[n]
+template <int THREADS, int SL>
+__device__ void block_excl_scan(uint32_t *v, int n, uint32_t *wsum) {
+    constexpr int PER = (SL + THREADS - 1) / THREADS;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    uint32_t loc[PER];
+    uint32_t sum = 0;
+#pragma unroll
+    for (int k = 0; k < PER; k++) {
+        const int i = tid * PER + k;
+        loc[k] = i < n ? v[i] : 0u;
+        sum += loc[k];
+    }
+    const uint32_t inc = wave_incl_scan(sum, lane);
+    if (lane == 63) wsum[wave] = inc;
+    __syncthreads();
+    uint32_t wbase = 0;
+    for (int w = 0; w < wave; w++) wbase += wsum[w];
+    uint32_t run = wbase + inc - sum;
+#pragma unroll
+    for (int k = 0; k < PER; k++) {
+        const int i = tid * PER + k;
+        if (i < n) v[i] = run;
+        run += loc[k];
+    }
+    if (tid == THREADS - 1) v[n] = run;
+    __syncthreads();
+}
+
+template <int THREADS, int TILE_WORDS, int SL>
+__global__ void __launch_bounds__(THREADS) search_items_kernel(ItemArgs a) {
+    constexpr int NW = THREADS / 64;
+    __shared__ ItemLds<THREADS, TILE_WORDS, SL> L;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int dF = a.dF, nb5 = 2 * dF + 1;
+    const uint32_t min_cnt = (uint32_t)max(a.min_shp, 1);
+    uint32_t *g_cand = a.cand_scratch + (size_t)blockIdx.x * a.tile_entries_max;
+    for (int w = tid; w < TILE_WORDS; w += THREADS) L.cnt[w] = 0;
+    for (int w = tid; w < GX_HIST_BINS; w += THREADS) L.hist[w] = 0;
+    unsigned long long my_cand = 0;
+
+    for (;;) {
+        if (tid == 0) L.item = a.item0 + atomicAdd(a.work_counter, 1u);
+        __syncthreads();
+        const uint32_t item = L.item;
+        if (item >= a.item1) break;
+        const uint32_t s = a.items[2 * (uint64_t)item];
+        const uint32_t bt = a.items[2 * (uint64_t)item + 1];
+        const uint32_t b = bt & 0xffu, t = bt >> 8;
+        const uint32_t pi = s * (uint32_t)a.n_buckets + b;
+        const int32_t lo = a.pair_lo[pi], hi = a.pair_hi[pi];
+        const uint32_t tiles = a.pair_tiles[pi];
+        const uint32_t pack = a.pair_pack[pi];
+        const uint32_t lp = pack == 4 ? 2u : (pack == 2 ? 1u : 0u);  // log2 entries per word
+        const uint32_t fbits = 32u >> lp;                             // counter width
+        const uint32_t fmask = fbits == 32 ? 0xffffffffu : ((1u << fbits) - 1u);
+        const uint32_t T = (uint32_t)TILE_WORDS << lp;
+        const int32_t tlo = lo + (int32_t)(t * T);
+        const int32_t thi = min(hi, tlo + (int32_t)T - 1);
+        const uint32_t nent = (uint32_t)(thi - tlo + 1);
+        const DevBucket bk = a.buckets[b];
+        const uint32_t speclen = bk.speclen;
+        const double inv_d = 1.0 / (double)speclen;
+        const uint32_t tlo_id = (uint32_t)tlo * speclen;
+        const uint32_t thi_id = (uint32_t)(thi + 1) * speclen;
+        const int P = (int)a.n_legal[s];
+        const uint32_t qoff = a.qidx[s];
+        for (int k = tid; k < P; k += THREADS) {
+            L.pk_q[k] = a.lq[qoff + k];
+            L.pk_i[k] = a.li[qoff + k];
+        }
+        const int nsl = P * nb5;
+        const uint64_t pbase = (a.pos_off[s] - a.pos_off[a.chunk_s0]) + a.pair_pos_off[pi];
+        const uint32_t *ia = bk.iA;
+        // ---- pass 1: shared-peak counts of every entry in the tile ----
+        for (int c0 = 0; c0 < nsl; c0 += SL) {
+            const int ns = min(SL, nsl - c0);
+            for (int j = tid; j < ns; j += THREADS) {
+                const uint32_t a0 = a.pos[pbase + 2 * (uint64_t)(c0 + j)];
+                const uint32_t z0 = a.pos[pbase + 2 * (uint64_t)(c0 + j) + 1];
+                uint32_t at = a0, zt = z0;
+                if (tiles > 1 && z0 > a0) {
+                    if (t > 0) at = a0 + lower_bound_u32(ia + a0, z0 - a0, tlo_id);
+                    if (t + 1 < tiles) zt = a0 + lower_bound_u32(ia + a0, z0 - a0, thi_id);
+                }
+                L.s_a[j] = at;
+                L.s_n[j] = zt - at;
+                L.s_off[j] = (zt - at + 63) >> 6;
+            }
+            __syncthreads();
+            block_excl_scan<THREADS, SL>(L.s_off, ns, L.wsum);
+            const uint32_t np = L.s_off[ns];
+            const uint32_t per = (np + NW - 1) / NW;
+            const uint32_t q0 = min(np, (uint32_t)wave * per), q1 = min(np, q0 + per);
+            if (q0 < q1) {
+                int j = upper_bound_lds(L.s_off, ns + 1, q0) - 1;
+                for (uint32_t q = q0; q < q1; q += kUnroll) {
+                    uint32_t raw[kUnroll];
+                    bool ok[kUnroll];
+#pragma unroll
+                    for (int u = 0; u < kUnroll; u++) {
+                        const uint32_t qq = q + u;
+                        ok[u] = false;
+                        raw[u] = 0;
+                        if (qq < q1) {
+                            while (L.s_off[j + 1] <= qq) j++;
+                            const uint32_t k = (qq - L.s_off[j]) << 6;
+                            const uint32_t pos = k + (uint32_t)lane;
+                            if (pos < L.s_n[j]) {
+                                ok[u] = true;
+                                raw[u] = ia[L.s_a[j] + pos];
+                            }
+                        }
+                    }
+#pragma unroll
+                    for (int u = 0; u < kUnroll; u++) {
+                        if (ok[u]) {
+                            const uint32_t el = div_entry(raw[u], inv_d) - (uint32_t)tlo;
+                            atomicAdd(&L.cnt[el >> lp], 1u << ((el & (pack - 1)) * fbits));
+                        }
+                    }
+                }
+            }
+            __syncthreads();
+        }
+        // ---- candidates: entries with count >= min_shp; clear the tile ----
+        if (tid == 0) L.n_cand = 0;
+        __syncthreads();
+        const uint32_t nw = (nent + pack - 1) >> lp;
+        for (uint32_t w = tid; w < nw; w += THREADS) {
+            const uint32_t v = L.cnt[w];
+            if (v == 0) continue;
+            L.cnt[w] = 0;
+            for (uint32_t e = 0; e < pack; e++) {
+                const uint32_t c = (v >> (e * fbits)) & fmask;
+                const uint32_t el = (w << lp) + e;
+                if (c >= min_cnt && el < nent) {
+                    const uint32_t pos = atomicAdd(&L.n_cand, 1u);
+                    const uint32_t psid = (uint32_t)tlo + el;
+                    if (pos < (uint32_t)kCandLds)
+                        L.cand[pos] = psid;
+                    else
+                        g_cand[pos - kCandLds] = psid;
+                }
+            }
+        }
+        __syncthreads();
+        // ---- pass 2: exact bc/yc/ibc/iyc per candidate, one wave each ----
+        const uint32_t ncand = L.n_cand;
+        const uint32_t half = speclen >> 1;
+        float bh = 0.0f;
+        int bb = 0x7fffffff, bp = 0x7fffffff;
+        uint32_t bshp = 0, beq = 0, bcps = 0;
+        for (uint32_t ci = wave; ci < ncand; ci += NW) {
+            const uint32_t psid = ci < (uint32_t)kCandLds ? L.cand[ci] : g_cand[ci - kCandLds];
+            const uint32_t *bins = bk.ion_bin + (size_t)psid * speclen;
+            uint32_t bc = 0, yc = 0, ibc = 0, iyc = 0;
+            for (uint32_t r = lane; r < speclen; r += 64) {
+                const int32_t x = (int32_t)bins[r];
+                int k = lower_bound_lds(L.pk_q, P, x - dF);
+                uint32_t n = 0, sum = 0;
+                for (; k < P && L.pk_q[k] <= x + dF; k++) {
+                    n++;
+                    sum += (uint32_t)L.pk_i[k];
+                }
+                if (r < half) {
+                    bc += n;
+                    ibc += sum;
+                } else {
+                    yc += n;
+                    iyc += sum;
+                }
+            }
+            bc = wave_sum_u32(bc);
+            yc = wave_sum_u32(yc);
+            ibc = wave_sum_u32(ibc);
+            iyc = wave_sum_u32(iyc);
+            if (lane == 0) {
+                const uint32_t shpk = bc + yc;
+                if (shpk >= (uint32_t)a.min_shp && (int)bc < a.lgfact_n && (int)yc < a.lgfact_n) {
+                    float hs = hyperscore(bc, yc, ibc, iyc, a.lgfact);
+                    if (hs > 0) {
+                        if (hs >= 80.0f) hs = 80.0f - 1;
+                        bcps++;
+                        atomicAdd(&L.hist[hyp_bin(hs)], 1u);
+                        if (better(hs, (int)b, (int)psid, bh, bb, bp)) {
+                            if (hs != bh) beq = 0;
+                            bh = hs;
+                            bb = (int)b;
+                            bp = (int)psid;
+                            bshp = shpk;
+                        }
+                        if (hs == bh) beq++;
+                    }
+                }
+            }
+        }
+        if (lane == 0) {
+            L.w_key[wave] = bcps ? hit_key(bh, (uint32_t)bb, (uint32_t)bp) : 0ull;
+            L.w_eq[wave] = beq;
+            L.w_shp[wave] = bshp;
+            L.w_cps[wave] = bcps;
+        }
+        __syncthreads();
+        // ---- flush to the spectrum's accumulators ----
+        for (int w = tid; w < GX_HIST_BINS; w += THREADS) {
+            const uint32_t v = L.hist[w];
+            if (v) {
+                atomicAdd(&a.hist[(size_t)s * GX_HIST_BINS + w], v);
+                L.hist[w] = 0;
+            }
+        }
+        if (tid == 0) {
+            unsigned long long key = 0;
+            uint32_t eq = 0, shp = 0, cps = 0;
+            for (int w = 0; w < NW; w++) {
+                cps += L.w_cps[w];
+                if (L.w_key[w] > key) {
+                    key = L.w_key[w];
+                    shp = L.w_shp[w];
+                }
+            }
+            for (int w = 0; w < NW; w++)
+                if (L.w_key[w] && (L.w_key[w] >> 32) == (key >> 32)) eq += L.w_eq[w];
+            a.item_key[item] = key;
+            a.item_eq[item] = eq;
+            a.item_shp[item] = shp;
+            if (cps) atomicAdd(&a.cpsms[s], cps);
+            my_cand += cps;
+        }
+        __syncthreads();
+    }
+    if (tid == 0) stat_add(a.stats, ST_CAND, my_cand);
+}
+
+// ---------------------------------------------------------------------------
+// one 64-thread block per spectrum
+__global__ void __launch_bounds__(64) evalue_kernel(EvalArgs a) {
+    __shared__ uint32_t hist[GX_HIST_BINS];
+    __shared__ double sx[GX_HIST_BINS];
+    __shared__ FitScratch F;
+    const int lane = threadIdx.x;
+    const uint32_t s = blockIdx.x;
+    const uint32_t i0 = a.item_off[s], i1 = a.item_off[s + 1];
+    unsigned long long key = 0;
+    for (uint32_t i = i0 + lane; i < i1; i += 64) {
+        const unsigned long long k = a.item_key[i];
+        key = k > key ? k : key;
+    }
+    key = wave_max_u64(key);
+    uint32_t eq = 0, shp = 0;
+    for (uint32_t i = i0 + lane; i < i1; i += 64) {
+        const unsigned long long k = a.item_key[i];
+        if (k && (k >> 32) == (key >> 32)) eq += a.item_eq[i];
+        if (k && k == key) shp = a.item_shp[i];
+    }
+    eq = wave_sum_u32(eq);
+    shp = (uint32_t)wave_max_i32((int)shp);
+    for (int w = lane; w < GX_HIST_BINS; w += 64) hist[w] = a.hist[(size_t)s * GX_HIST_BINS + w];
+    __syncthreads();
+    const uint32_t cps = a.cpsms[s];
+    gx_psm r;
+    r.hyperscore = 0.0f;
+    r.psid = -1;
+    r.idxoffset = 0;
+    r.sharedions = 0;
+    r.totalions = 0;
+    r.flags = 0;
+    r.cpsms = (int32_t)cps;
+    r.hyp = 0;
+    r.mu = 0;
+    r.beta = 0;
+    r.evalue = INFINITY;
+    if (cps >= 1 && key) {
+        const uint32_t low = ~(uint32_t)(key & 0xffffffffu);
+        const uint32_t b = low >> 26, psid = low & ((1u << 26) - 1);
+        r.flags |= GX_PSM_HAS_HIT;
+        r.hyperscore = __uint_as_float((uint32_t)(key >> 32));
+        r.psid = (int32_t)psid;
+        r.idxoffset = (uint16_t)b;
+        r.sharedions = (uint16_t)shp;
+        r.totalions = (uint16_t)a.buckets[b].speclen;
+        r.hyp = hyp_bin(r.hyperscore);
+        if (eq > 1) r.flags |= GX_PSM_TIE;
+    }
+    const uint32_t need = (uint32_t)max(a.min_cpsm, 1);
+    if (cps >= need && r.psid >= 0) {
+        r.flags |= GX_PSM_MODELED;
+        double sl, ic;
+        int32_t mu, beta;
+        if (tail_fit_block(hist, sx, F, r.hyp, (int)cps, sl, ic, mu, beta)) {
+            r.flags |= GX_PSM_FIT_OK;
+            r.mu = mu;
+            r.beta = beta;
+            r.evalue = evalue_of(mu, beta, r.hyp, cps);
+            if (r.evalue < a.expect_max) r.flags |= GX_PSM_REPORTED;
+        }
+    }
+    if (lane == 0) a.out[s] = r;
+}
+
+template <int THREADS, int TILE_WORDS, int SL>
+hipError_t launch_items(const ItemArgs &a, int n_blocks, hipStream_t s) {
+    hipLaunchKernelGGL((search_items_kernel<THREADS, TILE_WORDS, SL>), dim3(n_blocks), dim3(THREADS), 0, s, a);
+    return hipGetLastError();
+}
+
+}  // namespace
+
+// LDS per block: 4*TILE_WORDS + 16 KB peaks + 12*SL + 4 KB candidates + 3.2 KB histogram
+BatchConfig batch_config(int id) {
+    switch (id) {
+        case 0: return {1024, 28672, 1024, 1};  // 112 KB tile, one block per CU
+        case 1: return {512, 12288, 512, 2};    // 48 KB tile, two blocks per CU
+        default: return {256, 4096, 256, 3};    // 16 KB tile, three blocks per CU
+    }
+}
+
+hipError_t launch_fill_u32(uint32_t *p, uint64_t n, uint32_t v, hipStream_t s) {
+    if (n == 0) return hipSuccess;
+    uint64_t blocks = (n + 255) / 256;
+    if (blocks > 8192) blocks = 8192;
+    hipLaunchKernelGGL(fill_u32_kernel, dim3((unsigned)blocks), dim3(256), 0, s, p, n, v);
+    return hipGetLastError();
+}
+
+hipError_t launch_plan_spectra(const PlanArgs &a, hipStream_t s) {
+    if (a.n_spectra == 0) return hipSuccess;
+    hipLaunchKernelGGL(plan_spectra_kernel, dim3(a.n_spectra), dim3(kPlanThreads), 0, s, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_scan(const PlanArgs &a, hipStream_t s) {
+    hipLaunchKernelGGL(scan_kernel, dim3(1), dim3(1024), 0, s, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_plan_slices(const PlanArgs &a, hipStream_t s) {
+    const uint64_t nblk = (uint64_t)(a.chunk_s1 - a.chunk_s0) * (uint64_t)a.n_buckets;
+    if (nblk == 0) return hipSuccess;
+    hipLaunchKernelGGL(plan_slices_kernel, dim3((unsigned)nblk), dim3(kPlanThreads), 0, s, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_search_items(int config, const ItemArgs &a, int n_blocks, hipStream_t s) {
+    switch (config) {
+        case 0: return launch_items<1024, 28672, 1024>(a, n_blocks, s);
+        case 1: return launch_items<512, 12288, 512>(a, n_blocks, s);
+        default: return launch_items<256, 4096, 256>(a, n_blocks, s);
+    }
+}
+
+hipError_t launch_evalue(const EvalArgs &a, hipStream_t s) {
+    if (a.n_spectra == 0) return hipSuccess;
+    hipLaunchKernelGGL(evalue_kernel, dim3(a.n_spectra), dim3(64), 0, s, a);
+    return hipGetLastError();
+}
+
+}  // namespace gx

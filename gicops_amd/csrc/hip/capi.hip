@@ -10,6 +10,7 @@
 #include <new>
 #include <vector>
 
+#include "batched.h"
 #include "gicops_amd.h"
 #include "search_common.h"
 
@@ -20,7 +21,9 @@ struct gx_ctx {
     int n_cu = 0;
     char name[256] = {};
     hipStream_t stream = nullptr;
-    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr, ev2 = nullptr, ev3 = nullptr;
+    std::vector<hipEvent_t> ev_chunk;  // start/stop per planning chunk
+    int tuned_config = 0;
     double *d_lgfact = nullptr;
     int lgfact_n = 0;
     uint32_t *d_counter = nullptr;
@@ -56,8 +59,27 @@ struct gx_dqueries {
     float *prec = nullptr;
     float *dm = nullptr;
     gx_psm *psm = nullptr;
-    uint32_t *hist = nullptr;  // allocated when histograms are requested
+    uint32_t *hist = nullptr;  // per-spectrum survival histograms (batched: always; plain: on request)
     bool searched = false;
+    // batched engine workspace, sized on first use
+    int nb = 0;
+    int32_t *lq = nullptr, *li = nullptr;
+    uint32_t *n_legal = nullptr;
+    int32_t *pair_lo = nullptr, *pair_hi = nullptr;
+    uint32_t *pair_tiles = nullptr, *pair_pack = nullptr, *pair_item_off = nullptr, *pair_pos_off = nullptr;
+    uint32_t *n_items = nullptr, *item_off = nullptr;
+    uint64_t *n_pos = nullptr, *pos_off = nullptr;
+    uint32_t *cpsms = nullptr;
+    uint32_t *counters = nullptr;  // one work counter per chunk
+    size_t counters_n = 0;
+    size_t slots_cap = 0;          // per-item top-hit slots
+    uint32_t *items = nullptr;
+    size_t items_cap = 0;  // items
+    unsigned long long *item_key = nullptr;
+    uint32_t *item_eq = nullptr, *item_shp = nullptr;
+    uint32_t *pos = nullptr;
+    size_t pos_cap = 0;  // words
+    std::vector<uint64_t> h_pos_off;
 };
 
 namespace {
@@ -131,7 +153,9 @@ extern "C" int gx_ctx_create(int device, gx_ctx **out) {
     c->n_cu = prop.multiProcessorCount;
     std::snprintf(c->name, sizeof(c->name), "%s (%s)", prop.name, prop.gcnArchName);
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) return fail(GX_ERR_DEVICE);
-    if (hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess) return fail(GX_ERR_DEVICE);
+    if (hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess ||
+        hipEventCreate(&c->ev2) != hipSuccess || hipEventCreate(&c->ev3) != hipSuccess)
+        return fail(GX_ERR_DEVICE);
     // lgfact[i] = sum_{j=2..i} log10(j) (utils.h:115-131), computed on the host
     // with the same libm the CPU search uses, long enough for any count
     c->lgfact_n = 4096;
@@ -140,7 +164,7 @@ extern "C" int gx_ctx_create(int device, gx_ctx **out) {
     lg[1] = std::log10(1.0);
     for (int i = 2; i < c->lgfact_n; i++) lg[i] = std::log10((double)i) + lg[i - 1];
     if (dalloc(&c->d_lgfact, lg.size())) return fail(GX_ERR_BAD_MEM_ALLOC);
-    if (dalloc(&c->d_counter, 1) || dalloc(&c->d_stats, ST_N)) return fail(GX_ERR_BAD_MEM_ALLOC);
+    if (dalloc(&c->d_counter, 1) || dalloc(&c->d_stats, (size_t)kStatRows * ST_N)) return fail(GX_ERR_BAD_MEM_ALLOC);
     if (hipMemcpy(c->d_lgfact, lg.data(), lg.size() * sizeof(double), hipMemcpyHostToDevice) != hipSuccess)
         return fail(GX_ERR_DEVICE);
     *out = c;
@@ -156,8 +180,11 @@ extern "C" void gx_ctx_destroy(gx_ctx *c) {
     if (c->d_stats) (void)hipFree(c->d_stats);
     if (c->d_slices) (void)hipFree(c->d_slices);
     if (c->d_cand) (void)hipFree(c->d_cand);
-    if (c->ev0) (void)hipEventDestroy(c->ev0);
-    if (c->ev1) (void)hipEventDestroy(c->ev1);
+    hipEvent_t evs[] = {c->ev0, c->ev1, c->ev2, c->ev3};
+    for (hipEvent_t e : evs)
+        if (e) (void)hipEventDestroy(e);
+    for (hipEvent_t e : c->ev_chunk)
+        if (e) (void)hipEventDestroy(e);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
 }
@@ -197,6 +224,7 @@ extern "C" int gx_index_upload(gx_ctx *c, const gx_bucket *b, int nb, int maxz, 
         if (x.speclen != (uint32_t)((x.peplen - 1) * maxz * 2) || x.speclen < 2) return fail(GX_ERR_INVLD_INDEX);
         if ((uint64_t)x.n_entries * x.speclen != x.n_ions || x.n_ions >= (1ull << 32)) return fail(GX_ERR_INVLD_SIZE);
         if (x.bA[x.nbins] != x.n_ions) return fail(GX_ERR_INVLD_INDEX);
+        if (x.n_entries >= (1u << 26)) return fail(GX_ERR_INVLD_SIZE);  // packed top-hit key holds 26-bit psids
     }
     d->nbins = nbins;
     for (int i = 0; i < nb; i++) {
@@ -294,28 +322,56 @@ extern "C" void gx_queries_free(gx_dqueries *d) {
     if (!d) return;
     (void)hipSetDevice(d->ctx->device);
     (void)hipStreamSynchronize(d->ctx->stream);
-    void *ps[] = {d->idx, d->moz, d->inten, d->prec, d->dm, d->psm, d->hist};
+    void *ps[] = {d->idx,       d->moz,        d->inten,         d->prec,         d->dm,       d->psm,
+                  d->hist,      d->lq,         d->li,            d->n_legal,      d->pair_lo,  d->pair_hi,
+                  d->pair_tiles, d->pair_pack, d->pair_item_off, d->pair_pos_off, d->n_items,  d->item_off,
+                  d->n_pos,     d->pos_off,    d->cpsms,         d->counters,     d->items,    d->item_key,
+                  d->item_eq,   d->item_shp,   d->pos};
     for (void *p : ps)
         if (p) (void)hipFree(p);
     delete d;
 }
 
-static int search_locked(gx_ctx *c, const gx_dindex *idx, gx_dqueries *q, const gx_search_params *p,
-                         gx_search_stats *stats, bool want_hist) {
-    int st = check_params(p, idx);
-    if (st) return st;
-    if (idx->ctx != c || q->ctx != c) return GX_ERR_INVLD_PARAM;
-    if (hipSetDevice(c->device) != hipSuccess) return GX_ERR_DEVICE;
+namespace {
+
+void read_stats(gx_ctx *c, unsigned long long *h) {
+    std::vector<unsigned long long> rows((size_t)kStatRows * ST_N);
+    for (int k = 0; k < ST_N; k++) h[k] = 0;
+    if (hipMemcpy(rows.data(), c->d_stats, rows.size() * sizeof(unsigned long long), hipMemcpyDeviceToHost) !=
+        hipSuccess)
+        return;
+    for (int r = 0; r < kStatRows; r++)
+        for (int k = 0; k < ST_N; k++) h[k] += rows[(size_t)r * ST_N + k];
+}
+
+void fill_stats(gx_search_stats *stats, const unsigned long long *h) {
+    std::memset(stats, 0, sizeof(*stats));
+    stats->ions = h[ST_IONS];
+    stats->bins = h[ST_BINS];
+    stats->window = h[ST_WINDOW];
+    stats->candidates = h[ST_CAND];
+    stats->spectra = h[ST_SPECTRA];
+    stats->peaks = h[ST_PEAKS];
+    stats->fallbacks = h[ST_FALLBACK];
+}
+
+float elapsed(hipEvent_t a, hipEvent_t b) {
+    float ms = 0;
+    if (hipEventElapsedTime(&ms, a, b) != hipSuccess) return 0;
+    return ms;
+}
+
+// one block per spectrum, tail fit fused (search.hip)
+int search_plain(gx_ctx *c, const gx_dindex *idx, gx_dqueries *q, const gx_search_params *p,
+                 gx_search_stats *stats, bool want_hist) {
     if (want_hist && !q->hist) {
-        st = dalloc(&q->hist, (size_t)q->n_spectra * GX_HIST_BINS);
+        int st = dalloc(&q->hist, (size_t)q->n_spectra * GX_HIST_BINS);
         if (st) return st;
     }
     const uint32_t nb5 = (uint32_t)(2 * p->dF + 1);
     const uint32_t max_slices = std::max<uint32_t>(1, q->max_peaks * nb5);
-    // every count bc or yc stays below (2dF+1)*speclen/2 + 1
-    if ((int)(nb5 * (idx->max_speclen / 2) + 1) > c->lgfact_n) return GX_ERR_INVLD_SIZE;
     const int n_blocks = std::max(1, c->n_cu * 2);
-    st = ensure_scratch(c, n_blocks, max_slices);
+    int st = ensure_scratch(c, n_blocks, max_slices);
     if (st) return st;
     SearchArgs a{};
     a.buckets = idx->d_buckets;
@@ -341,6 +397,7 @@ static int search_locked(gx_ctx *c, const gx_dindex *idx, gx_dqueries *q, const 
     a.cand_scratch = c->d_cand;
     a.max_slices = max_slices;
     a.stats = c->d_stats;
+    GX_HIP(launch_fill_u32(reinterpret_cast<uint32_t *>(c->d_stats), (uint64_t)kStatRows * ST_N * 2, 0, c->stream));
     GX_HIP(launch_reset(c->d_counter, c->d_stats, c->stream));
     GX_HIP(hipEventRecord(c->ev0, c->stream));
     if (q->n_spectra > 0) GX_HIP(launch_search(a, n_blocks, c->stream));
@@ -349,22 +406,240 @@ static int search_locked(gx_ctx *c, const gx_dindex *idx, gx_dqueries *q, const 
     q->searched = true;
     if (stats) {
         unsigned long long h[ST_N];
-        GX_HIP(hipMemcpy(h, c->d_stats, sizeof(h), hipMemcpyDeviceToHost));
-        float ms = 0;
-        GX_HIP(hipEventElapsedTime(&ms, c->ev0, c->ev1));
-        std::memset(stats, 0, sizeof(*stats));
-        stats->ions = h[ST_IONS];
-        stats->bins = h[ST_BINS];
-        stats->window = h[ST_WINDOW];
-        stats->candidates = h[ST_CAND];
-        stats->spectra = h[ST_SPECTRA];
-        stats->peaks = h[ST_PEAKS];
-        stats->fallbacks = h[ST_FALLBACK];
-        stats->kernel_ms = ms;
-        stats->evalue_ms = 0;
+        read_stats(c, h);
+        fill_stats(stats, h);
+        stats->kernel_ms = elapsed(c->ev0, c->ev1);
+        stats->total_ms = stats->kernel_ms;
+        stats->config = -1;
     }
     return GX_SUCCESS;
 }
+
+// size the per-spectrum workspace of the batched engine
+int ensure_batched(gx_dqueries *q, const gx_dindex *idx) {
+    if (q->lq && q->nb == idx->n_buckets) return GX_SUCCESS;
+    if (q->lq) return GX_ERR_INVLD_PARAM;  // a batch serves one index shape
+    const size_t ns = q->n_spectra, np = q->n_peaks, npairs = ns * (size_t)idx->n_buckets;
+    q->nb = idx->n_buckets;
+    if (dalloc(&q->lq, np) || dalloc(&q->li, np) || dalloc(&q->n_legal, ns) || dalloc(&q->pair_lo, npairs) ||
+        dalloc(&q->pair_hi, npairs) || dalloc(&q->pair_tiles, npairs) || dalloc(&q->pair_pack, npairs) ||
+        dalloc(&q->pair_item_off, npairs) || dalloc(&q->pair_pos_off, npairs) || dalloc(&q->n_items, ns) ||
+        dalloc(&q->item_off, ns + 1) || dalloc(&q->n_pos, ns) || dalloc(&q->pos_off, ns + 1) ||
+        dalloc(&q->cpsms, ns) || (!q->hist && dalloc(&q->hist, ns * GX_HIST_BINS)))
+        return GX_ERR_BAD_MEM_ALLOC;
+    q->h_pos_off.resize(ns + 1);
+    return GX_SUCCESS;
+}
+
+template <class T>
+int grow(T **p, size_t &cap, size_t n) {
+    if (n <= cap && *p) return GX_SUCCESS;
+    if (*p) (void)hipFree(*p);
+    *p = nullptr;
+    cap = 0;
+    size_t want = n + n / 8 + 64;
+    int st = dalloc(p, want);
+    if (!st) cap = want;
+    return st;
+}
+
+constexpr uint64_t kPosChunkWords = 1ull << 27;  // 512 MB of slice positions per planning chunk
+
+int search_batched(gx_ctx *c, const gx_dindex *idx, gx_dqueries *q, const gx_search_params *p,
+                   gx_search_stats *stats) {
+    int st = ensure_batched(q, idx);
+    if (st) return st;
+    const int cfg_id = p->config >= 0 ? std::min(p->config, kNumBatchConfigs - 1) : c->tuned_config;
+    const BatchConfig cfg = batch_config(cfg_id);
+    const uint32_t ns = q->n_spectra;
+    PlanArgs pa{};
+    pa.buckets = idx->d_buckets;
+    pa.n_buckets = idx->n_buckets;
+    pa.qidx = q->idx;
+    pa.moz = q->moz;
+    pa.inten = q->inten;
+    pa.prec = q->prec;
+    pa.dm = q->has_dm ? q->dm : nullptr;
+    pa.n_spectra = ns;
+    pa.dF = p->dF;
+    pa.nbins = (int)idx->nbins;
+    pa.dM = p->dM;
+    pa.tile_words = cfg.tile_words;
+    pa.lq = q->lq;
+    pa.li = q->li;
+    pa.n_legal = q->n_legal;
+    pa.pair_lo = q->pair_lo;
+    pa.pair_hi = q->pair_hi;
+    pa.pair_tiles = q->pair_tiles;
+    pa.pair_pack = q->pair_pack;
+    pa.pair_item_off = q->pair_item_off;
+    pa.pair_pos_off = q->pair_pos_off;
+    pa.n_items = q->n_items;
+    pa.n_pos = q->n_pos;
+    pa.item_off = q->item_off;
+    pa.pos_off = q->pos_off;
+    pa.stats = c->d_stats;
+    hipStream_t sm = c->stream;
+    GX_HIP(launch_fill_u32(reinterpret_cast<uint32_t *>(c->d_stats), (uint64_t)kStatRows * ST_N * 2, 0, sm));
+    GX_HIP(launch_fill_u32(q->hist, (uint64_t)ns * GX_HIST_BINS, 0, sm));
+    GX_HIP(launch_fill_u32(q->cpsms, ns, 0, sm));
+    GX_HIP(hipEventRecord(c->ev0, sm));
+    GX_HIP(launch_plan_spectra(pa, sm));
+    GX_HIP(launch_scan(pa, sm));
+    GX_HIP(hipEventRecord(c->ev1, sm));
+    // chunking needs the slice-position offsets on the host
+    uint32_t n_items_total = 0;
+    GX_HIP(hipMemcpyAsync(q->h_pos_off.data(), q->pos_off, (ns + 1) * sizeof(uint64_t), hipMemcpyDeviceToHost, sm));
+    GX_HIP(hipMemcpyAsync(&n_items_total, q->item_off + ns, sizeof(uint32_t), hipMemcpyDeviceToHost, sm));
+    GX_HIP(hipStreamSynchronize(sm));
+    std::vector<uint32_t> bounds{0};
+    uint64_t max_chunk = 1;
+    {
+        uint32_t s0 = 0;
+        while (s0 < ns) {
+            uint32_t s1 = s0 + 1;
+            while (s1 < ns && q->h_pos_off[s1 + 1] - q->h_pos_off[s0] <= kPosChunkWords) s1++;
+            max_chunk = std::max<uint64_t>(max_chunk, q->h_pos_off[s1] - q->h_pos_off[s0]);
+            bounds.push_back(s1);
+            s0 = s1;
+        }
+    }
+    const size_t n_chunks = bounds.size() - 1;
+    if (grow(&q->pos, q->pos_cap, max_chunk) || grow(&q->items, q->items_cap, 2 * (size_t)n_items_total + 2))
+        return GX_ERR_BAD_MEM_ALLOC;
+    if (q->items_cap / 2 + 1 > q->slots_cap || !q->item_key) {
+        // per-item slots follow the item capacity
+        if (q->item_key) (void)hipFree(q->item_key);
+        if (q->item_eq) (void)hipFree(q->item_eq);
+        if (q->item_shp) (void)hipFree(q->item_shp);
+        q->item_key = nullptr;
+        q->item_eq = q->item_shp = nullptr;
+        const size_t n = q->items_cap / 2 + 1;
+        if (dalloc(&q->item_key, n) || dalloc(&q->item_eq, n) || dalloc(&q->item_shp, n)) return GX_ERR_BAD_MEM_ALLOC;
+        q->slots_cap = n;
+    }
+    if (!q->counters || n_chunks > q->counters_n) {
+        if (q->counters) (void)hipFree(q->counters);
+        q->counters = nullptr;
+        q->counters_n = 0;
+        if (dalloc(&q->counters, std::max<size_t>(64, n_chunks))) return GX_ERR_BAD_MEM_ALLOC;
+        q->counters_n = std::max<size_t>(64, n_chunks);
+    }
+    const int n_blocks = std::max(1, c->n_cu * cfg.blocks_per_cu);
+    const size_t tile_entries_max = (size_t)cfg.tile_words * 4;
+    if (ensure_scratch(c, n_blocks, 1)) return GX_ERR_BAD_MEM_ALLOC;
+    if (c->cand_words < (size_t)n_blocks * tile_entries_max) {
+        if (c->d_cand) (void)hipFree(c->d_cand);
+        c->d_cand = nullptr;
+        c->cand_words = 0;
+        if (dalloc(&c->d_cand, (size_t)n_blocks * tile_entries_max)) return GX_ERR_BAD_MEM_ALLOC;
+        c->cand_words = (size_t)n_blocks * tile_entries_max;
+    }
+    while (c->ev_chunk.size() < 2 * n_chunks) {
+        hipEvent_t e;
+        GX_HIP(hipEventCreate(&e));
+        c->ev_chunk.push_back(e);
+    }
+    GX_HIP(launch_fill_u32(q->counters, std::max<size_t>(64, n_chunks), 0, sm));
+    pa.pos = q->pos;
+    pa.items = q->items;
+    ItemArgs ia{};
+    ia.buckets = idx->d_buckets;
+    ia.n_buckets = idx->n_buckets;
+    ia.qidx = q->idx;
+    ia.dF = p->dF;
+    ia.min_shp = p->min_shp;
+    ia.lgfact = c->d_lgfact;
+    ia.lgfact_n = c->lgfact_n;
+    ia.lq = q->lq;
+    ia.li = q->li;
+    ia.n_legal = q->n_legal;
+    ia.pair_lo = q->pair_lo;
+    ia.pair_hi = q->pair_hi;
+    ia.pair_tiles = q->pair_tiles;
+    ia.pair_pack = q->pair_pack;
+    ia.pair_pos_off = q->pair_pos_off;
+    ia.pos_off = q->pos_off;
+    ia.pos = q->pos;
+    ia.items = q->items;
+    ia.cand_scratch = c->d_cand;
+    ia.tile_entries_max = (int)tile_entries_max;
+    ia.hist = q->hist;
+    ia.cpsms = q->cpsms;
+    ia.item_key = q->item_key;
+    ia.item_eq = q->item_eq;
+    ia.item_shp = q->item_shp;
+    ia.stats = c->d_stats;
+    std::vector<uint32_t> h_item_off(n_chunks + 1);
+    for (size_t k = 0; k <= n_chunks; k++)
+        GX_HIP(hipMemcpyAsync(&h_item_off[k], q->item_off + bounds[k], sizeof(uint32_t), hipMemcpyDeviceToHost, sm));
+    GX_HIP(hipStreamSynchronize(sm));
+    for (size_t k = 0; k < n_chunks; k++) {
+        pa.chunk_s0 = bounds[k];
+        pa.chunk_s1 = bounds[k + 1];
+        GX_HIP(hipEventRecord(c->ev_chunk[2 * k], sm));
+        GX_HIP(launch_plan_slices(pa, sm));
+        GX_HIP(hipEventRecord(c->ev_chunk[2 * k + 1], sm));
+        ia.item0 = h_item_off[k];
+        ia.item1 = h_item_off[k + 1];
+        ia.chunk_s0 = bounds[k];
+        ia.work_counter = q->counters + k;
+        if (ia.item1 > ia.item0) GX_HIP(launch_search_items(cfg_id, ia, n_blocks, sm));
+    }
+    GX_HIP(hipEventRecord(c->ev2, sm));
+    EvalArgs ea{};
+    ea.buckets = idx->d_buckets;
+    ea.n_spectra = ns;
+    ea.min_cpsm = p->min_cpsm;
+    ea.expect_max = p->expect_max;
+    ea.item_off = q->item_off;
+    ea.item_key = q->item_key;
+    ea.item_eq = q->item_eq;
+    ea.item_shp = q->item_shp;
+    ea.hist = q->hist;
+    ea.cpsms = q->cpsms;
+    ea.route = nullptr;
+    ea.out = q->psm;
+    ea.stats = c->d_stats;
+    GX_HIP(launch_evalue(ea, sm));
+    GX_HIP(hipEventRecord(c->ev3, sm));
+    GX_HIP(hipStreamSynchronize(sm));
+    q->searched = true;
+    if (stats) {
+        unsigned long long h[ST_N];
+        read_stats(c, h);
+        fill_stats(stats, h);
+        stats->items = n_items_total;
+        stats->chunks = (uint32_t)n_chunks;
+        stats->config = cfg_id;
+        double plan = elapsed(c->ev0, c->ev1), search = 0;
+        for (size_t k = 0; k < n_chunks; k++) {
+            plan += elapsed(c->ev_chunk[2 * k], c->ev_chunk[2 * k + 1]);
+            const hipEvent_t stop = (k + 1 < n_chunks) ? c->ev_chunk[2 * k + 2] : c->ev2;
+            search += elapsed(c->ev_chunk[2 * k + 1], stop);
+        }
+        stats->plan_ms = plan;
+        stats->kernel_ms = search;
+        stats->evalue_ms = elapsed(c->ev2, c->ev3);
+        stats->total_ms = elapsed(c->ev0, c->ev3);
+    }
+    return GX_SUCCESS;
+}
+
+int search_locked(gx_ctx *c, const gx_dindex *idx, gx_dqueries *q, const gx_search_params *p,
+                  gx_search_stats *stats, bool want_hist) {
+    int st = check_params(p, idx);
+    if (st) return st;
+    if (idx->ctx != c || q->ctx != c) return GX_ERR_INVLD_PARAM;
+    if (p->engine != GX_ENGINE_BATCHED && p->engine != GX_ENGINE_SPECTRUM) return GX_ERR_INVLD_PARAM;
+    if (hipSetDevice(c->device) != hipSuccess) return GX_ERR_DEVICE;
+    // every count bc or yc stays below (2dF+1)*speclen/2 + 1
+    if ((int)((2 * p->dF + 1) * (idx->max_speclen / 2) + 1) > c->lgfact_n) return GX_ERR_INVLD_SIZE;
+    if (p->engine == GX_ENGINE_SPECTRUM) return search_plain(c, idx, q, p, stats, want_hist);
+    return search_batched(c, idx, q, p, stats);
+}
+
+}  // namespace
 
 extern "C" int gx_search_device(gx_ctx *c, const gx_dindex *idx, gx_dqueries *q, const gx_search_params *p,
                                 gx_search_stats *stats) {

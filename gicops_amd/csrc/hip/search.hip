@@ -23,107 +23,13 @@
 //     then the hyperscore (double log10, stored as f32), the survival
 //     histogram (LDS atomics) and the top hit;
 //   * the tail fit runs on the LDS histogram at the end of the spectrum.
-#include "search_common.h"
-
-#include <math.h>
+#include "device_util.h"
 
 namespace gx {
 namespace {
 
 constexpr int kWaves = kSearchThreads / 64;
 constexpr uint32_t kLight = 16;  // slices with at most this many ions left are walked by one lane
-
-__device__ __forceinline__ uint32_t lower_bound_u32(const uint32_t *a, uint32_t n, uint32_t v) {
-    uint32_t lo = 0;
-    while (n > 0) {
-        uint32_t half = n >> 1;
-        if (a[lo + half] < v) {
-            lo += half + 1;
-            n -= half + 1;
-        } else {
-            n = half;
-        }
-    }
-    return lo;
-}
-
-__device__ __forceinline__ uint32_t lower_bound_f32(const float *a, uint32_t n, float v) {
-    uint32_t lo = 0;
-    while (n > 0) {
-        uint32_t half = n >> 1;
-        if (a[lo + half] < v) {
-            lo += half + 1;
-            n -= half + 1;
-        } else {
-            n = half;
-        }
-    }
-    return lo;
-}
-
-__device__ __forceinline__ uint32_t upper_bound_f32(const float *a, uint32_t n, float v) {
-    uint32_t lo = 0;
-    while (n > 0) {
-        uint32_t half = n >> 1;
-        if (!(v < a[lo + half])) {
-            lo += half + 1;
-            n -= half + 1;
-        } else {
-            n = half;
-        }
-    }
-    return lo;
-}
-
-// lower_bound over the LDS peak array (sorted m/z)
-__device__ __forceinline__ int lower_bound_lds(const int32_t *a, int n, int32_t v) {
-    int lo = 0;
-    while (n > 0) {
-        int half = n >> 1;
-        if (a[lo + half] < v) {
-            lo += half + 1;
-            n -= half + 1;
-        } else {
-            n = half;
-        }
-    }
-    return lo;
-}
-
-__device__ __forceinline__ uint32_t wave_sum_u32(uint32_t v) {
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-    return v;
-}
-
-__device__ __forceinline__ int wave_max_i32(int v) {
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) v = max(v, __shfl_xor(v, o, 64));
-    return v;
-}
-
-__device__ __forceinline__ int wave_min_i32(int v) {
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) v = min(v, __shfl_xor(v, o, 64));
-    return v;
-}
-
-// inclusive prefix sum across a wave
-__device__ __forceinline__ uint32_t wave_incl_scan(uint32_t v, int lane) {
-#pragma unroll
-    for (int o = 1; o < 64; o <<= 1) {
-        uint32_t t = __shfl_up(v, o, 64);
-        if (lane >= o) v += t;
-    }
-    return v;
-}
-
-// "better" order of top hits: higher f32 hyperscore, then smaller (bucket, psid)
-__device__ __forceinline__ bool better(float h, int b, int p, float bh, int bb, int bp) {
-    if (h != bh) return h > bh;
-    if (b != bb) return b < bb;
-    return p < bp;
-}
 
 struct Lds {
     uint32_t cnt[kTileWords];          // count tile; also scratch for the peak sort and the tail fit
@@ -145,163 +51,9 @@ struct Lds {
     uint32_t n_legal;
     uint32_t n_cand;
     uint32_t spec;
-    // tail fit
-    int32_t tf_i[8];
-    double tf_d[4];
+    FitScratch fit;
     unsigned long long st[ST_N];
 };
-
-// Tail fit (expeRT::ModelTailFit, expeRT.cpp:396-559) on the LDS histogram.
-// All threads call it; wave 0 does the work. The cumulative sum is over
-// integer counts, so its parallel order is exact; the regression sums run
-// sequentially in lane 0 in the reference's order. Returns fit_ok and writes
-// slope/intercept and the x1e6 truncated ints through `L.tf_*`.
-__device__ bool tail_fit(Lds &L, int hyp, int cpsms, double &slope, double &icept, int32_t &mu, int32_t &beta) {
-    const int tid = threadIdx.x, lane = tid & 63;
-    const bool w0 = tid < 64;
-    double *sx = reinterpret_cast<double *>(L.cnt);  // 802 doubles
-    if (hyp < 0 || hyp >= GX_HIST_BINS || cpsms < 1) {
-        slope = icept = 0;
-        mu = beta = 0;
-        return false;
-    }
-    // end1 = last p <= hyp-1 with count >= 1 (default hyp-1); stt1 = first p <= end1 with count >= 1 (default 0)
-    if (w0) {
-        int e = -1;
-        for (int p = lane; p <= hyp - 1; p += 64)
-            if (L.hist[p] >= 1) e = max(e, p);
-        e = wave_max_i32(e);
-        const int end1 = (e >= 0) ? e : hyp - 1;
-        int s = 0x7fffffff;
-        for (int p = lane; p <= end1; p += 64)
-            if (L.hist[p] >= 1) s = min(s, p);
-        s = wave_min_i32(s);
-        const int stt1 = (s != 0x7fffffff) ? s : 0;
-        if (lane == 0) {
-            L.tf_i[0] = stt1;
-            L.tf_i[1] = (stt1 == end1) ? end1 + 1 : end1;
-        }
-    }
-    __syncthreads();
-    const int stt1 = L.tf_i[0], end1 = L.tf_i[1];
-    const int n = end1 - stt1 + 1;
-    if (n < 1) {
-        slope = icept = 0;
-        mu = beta = 0;
-        return false;
-    }
-    const double vaa = (double)cpsms;
-    // s(x) = -((cumsum / vaa) + -1), clamped to 0.999 above 1
-    if (w0) {
-        uint32_t carry = 0;
-        for (int base = 0; base < n; base += 64) {
-            const int i = base + lane;
-            uint32_t v = (i < n && stt1 + i < GX_HIST_BINS) ? L.hist[stt1 + i] : 0u;
-            uint32_t inc = wave_incl_scan(v, lane) + carry;
-            if (i < n) {
-                double x = (double)inc;
-                x /= vaa;
-                x += -1.0;
-                x = -x;
-                if (x > 1.0) x = 0.999;
-                sx[i] = x;
-            }
-            carry = __shfl(inc, 63, 64);
-        }
-    }
-    __syncthreads();
-    // replacement = last index with s(x) >= 1e-4 (default n-1); values <= 0 take its value; log10
-    if (w0) {
-        int r = -1;
-        for (int i = lane; i < n; i += 64)
-            if (sx[i] >= 1e-4) r = max(r, i);
-        r = wave_max_i32(r);
-        const int repl = (r >= 0) ? r : n - 1;
-        const double rv = sx[repl];
-        __builtin_amdgcn_wave_barrier();
-        for (int i = lane; i < n; i += 64) {
-            double x = sx[i];
-            if (x <= 0.0) x = rv;
-            sx[i] = log10(x);
-        }
-    }
-    __syncthreads();
-    // marks at 22% and 87% of the height
-    if (w0) {
-        const double s0 = sx[0];
-        const double hgt = sx[n - 1] - s0;
-        int mark = 0, mark2 = 0;
-        if (n >= 3) {
-            const double t1 = s0 + hgt * 0.22;
-            int f = 0x7fffffff;
-            for (int i = lane; i < n; i += 64)
-                if (sx[i] <= t1) f = min(f, i);
-            f = wave_min_i32(f);
-            mark = ((f != 0x7fffffff) ? f : 0) - 1;
-            if (n > 3) {
-                const double t2 = s0 + hgt * 0.87;
-                int g = -1;
-                for (int i = lane; i < n; i += 64)
-                    if (sx[i] >= t2) g = max(g, i);
-                g = wave_max_i32(g);
-                mark2 = (g >= 0) ? g : n - 1;
-                if (mark2 == n) mark2 -= 1;
-            } else {
-                mark2 = n - 1;
-            }
-            if (mark >= mark2) mark = mark2 - 1;
-        } else {
-            mark = 0;
-            mark2 = n - 1;
-        }
-        if (lane == 0) {
-            L.tf_i[2] = mark;
-            L.tf_i[3] = mark2;
-            int ok = 0;
-            double a = 0, b = 0;
-            if (mark >= 0 && mark2 >= mark) {
-                const int m = mark2 - mark + 1;
-                if (m == 1) {
-                    a = 0.0;
-                    b = sx[mark];
-                } else {
-                    double xbar = 0.0, ybar = 0.0;
-                    for (int i = 0; i < m; i++) {
-                        xbar = xbar + (double)(stt1 + mark + i);
-                        ybar = ybar + sx[mark + i];
-                    }
-                    xbar = xbar / (double)m;
-                    ybar = ybar / (double)m;
-                    double top = 0.0, bot = 0.0;
-                    for (int i = 0; i < m; i++) {
-                        const double xi = (double)(stt1 + mark + i);
-                        top = top + (xi - xbar) * (sx[mark + i] - ybar);
-                        bot = bot + (xi - xbar) * (xi - xbar);
-                    }
-                    a = top / bot;
-                    b = ybar - a * xbar;
-                }
-                const double am = a * 1e6, bm = b * 1e6;
-                ok = (fabs(am) < 2147483647.0 && fabs(bm) < 2147483647.0) ? 1 : 0;
-                L.tf_i[5] = ok ? (int32_t)am : 0;
-                L.tf_i[6] = ok ? (int32_t)bm : 0;
-            }
-            L.tf_i[4] = ok;
-            L.tf_d[0] = a;
-            L.tf_d[1] = b;
-        }
-    }
-    __syncthreads();
-    const bool ok = L.tf_i[4] != 0;
-    slope = L.tf_d[0];
-    icept = L.tf_d[1];
-    mu = ok ? L.tf_i[5] : 0;
-    beta = ok ? L.tf_i[6] : 0;
-    __syncthreads();
-    // the fit used the count tile as scratch: clear what it touched
-    for (int w = tid; w < 2 * GX_HIST_BINS; w += kSearchThreads) L.cnt[w] = 0;
-    return ok;
-}
 
 __global__ void reset_kernel(uint32_t *counter, unsigned long long *stats) {
     const int t = threadIdx.x;
@@ -629,16 +381,15 @@ __global__ void __launch_bounds__(kSearchThreads, 4) search_kernel(SearchArgs a)
             double sl, ic;
             int32_t mu, beta;
             r.flags |= GX_PSM_MODELED;
-            if (tail_fit(L, r.hyp, (int)cps, sl, ic, mu, beta)) {
+            const bool fit_ok = tail_fit_block(L.hist, reinterpret_cast<double *>(L.cnt), L.fit, r.hyp, (int)cps, sl,
+                                               ic, mu, beta);
+            // the fit used the count tile as scratch: clear what it touched
+            for (int w = tid; w < 2 * GX_HIST_BINS; w += kSearchThreads) L.cnt[w] = 0;
+            if (fit_ok) {
                 r.flags |= GX_PSM_FIT_OK;
                 r.mu = mu;
                 r.beta = beta;
-                double w = mu, bb2 = beta;
-                w /= 1e6;
-                bb2 /= 1e6;
-                const double lgs = (w * r.hyp) + bb2;
-                double e = pow(10.0, lgs);
-                e *= (double)cps;
+                const double e = evalue_of(mu, beta, r.hyp, cps);
                 r.evalue = e;
                 if (e < a.expect_max) r.flags |= GX_PSM_REPORTED;
             }

@@ -33,12 +33,16 @@ class SearchParams:
     expect_max: float = 20.0
     evalue_mode: int = F.GX_EVALUE_TAILFIT
     surrogate_conf: float = 0.5
+    engine: int = F.GX_ENGINE_BATCHED
+    config: int = -1
+    use_graph: bool = False
 
     def to_ffi(self) -> F.gx_search_params:
         p = F.gx_search_params()
         p.dF, p.scale, p.max_mass = self.dF, self.scale, self.max_mass
         p.min_shp, p.min_cpsm, p.evalue_mode = self.min_shp, self.min_cpsm, self.evalue_mode
         p.dM, p.expect_max, p.surrogate_conf = self.dM, self.expect_max, self.surrogate_conf
+        p.engine, p.config, p.use_graph = self.engine, self.config, int(self.use_graph)
         return p
 
 

@@ -94,6 +94,9 @@ typedef struct gx_query_batch {
 
 /* ---- parameters (gParams subset, slm_dsts.h:349-431, argp.hpp:272-370) */
 enum { GX_EVALUE_TAILFIT = 0, GX_EVALUE_LEARNED = 1 };
+/* search engines: the batched plan + persistent item kernel (default), or one
+ * block per spectrum (the plain kernel) */
+enum { GX_ENGINE_BATCHED = 0, GX_ENGINE_SPECTRUM = 1 };
 
 typedef struct gx_search_params {
     int32_t dF;            /* fragment tolerance in bins ((uint)(0.02*scale) = 2) */
@@ -105,7 +108,9 @@ typedef struct gx_search_params {
     double dM;             /* precursor tolerance, Da (global) */
     double expect_max;     /* report gate: e < expect_max (20) */
     float surrogate_conf;  /* GX_EVALUE_LEARNED: confidence threshold for the MLP */
-    int32_t reserved;
+    int32_t engine;        /* GX_ENGINE_* */
+    int32_t config;        /* batched launch configuration, -1 = the context's tuned choice */
+    int32_t use_graph;     /* batched: 1 = capture the batch as a hipGraph once, then replay */
 } gx_search_params;
 
 /* ---- per-spectrum result (hCell + Results + e-value) ----------------- */
@@ -144,8 +149,13 @@ typedef struct gx_search_stats {
     uint64_t spectra;     /* spectra searched */
     uint64_t peaks;       /* query peaks read */
     uint64_t fallbacks;   /* learned mode: spectra routed to the plain tail fit */
-    double kernel_ms;     /* device time of the search kernels (HIP events) */
-    double evalue_ms;     /* device time of the e-value stage (0 if fused) */
+    uint64_t items;       /* batched: (spectrum, bucket, tile) work items */
+    uint32_t chunks;      /* batched: planning chunks */
+    int32_t config;       /* batched: launch configuration used */
+    double kernel_ms;     /* device time of the search kernel(s) (HIP events on the context stream) */
+    double evalue_ms;     /* device time of the e-value stage (0 when fused into the search kernel) */
+    double plan_ms;       /* device time of the plan kernels (batched engine) */
+    double total_ms;      /* device time of the whole call, first to last kernel */
 } gx_search_stats;
 
 typedef struct gx_ctx gx_ctx;

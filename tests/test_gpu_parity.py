@@ -43,6 +43,12 @@ def compare(gpu, cpu, hg=None, hc=None, label=""):
         assert np.array_equal(hg, hc), f"{label}: histograms differ"
 
 
+# every search engine and batched launch configuration
+ENGINES = [dict(engine=F.GX_ENGINE_BATCHED, config=0), dict(engine=F.GX_ENGINE_BATCHED, config=1),
+           dict(engine=F.GX_ENGINE_BATCHED, config=2), dict(engine=F.GX_ENGINE_SPECTRUM)]
+ENGINE_IDS = ["batched-c0", "batched-c1", "batched-c2", "spectrum"]
+
+
 def check_stats(st, cnt, n):
     assert st["spectra"] == n
     assert st["ions"] == int(cnt["ions"].sum())
@@ -50,9 +56,10 @@ def check_stats(st, cnt, n):
     assert st["window"] == int(cnt["window"].sum())
 
 
-def test_small_closed_search_parity(small_index, small_batch, searcher_small):
-    say("case: small closed search, 50k peptides, 1000 spectra, dM 10")
-    p = SearchParams(dM=10.0)
+@pytest.mark.parametrize("eng", ENGINES, ids=ENGINE_IDS)
+def test_small_closed_search_parity(small_index, small_batch, searcher_small, eng):
+    say(f"case: small closed search, 50k peptides, 1000 spectra, dM 10, {eng}")
+    p = SearchParams(dM=10.0, **eng)
     g, hg, st = searcher_small.search(small_batch, p, want_hist=True)
     c, hc, cnt = O.search(small_index, small_batch, O.default_params(dM=10.0), want_hist=True,
                           want_counters=True)
@@ -61,36 +68,39 @@ def test_small_closed_search_parity(small_index, small_batch, searcher_small):
     assert st["candidates"] == int(c["cpsms"].sum())
 
 
-def test_open_search_parity(open_index, searcher_open):
-    say("case: open search, 200k peptides + Met-ox, 300 spectra, dM 500")
+@pytest.mark.parametrize("eng", ENGINES, ids=ENGINE_IDS)
+def test_open_search_parity(open_index, searcher_open, eng):
+    say(f"case: open search, 200k peptides + Met-ox, 300 spectra, dM 500, {eng}")
     cfg = slm.SpecConfig(n_spectra=300, seed=21, peak_mode=F.GX_PEAKS_TOTAL_UNIFORM, peaks_min=50,
                          peaks_max=150, charge_min=2, charge_max=3, dm_choices=(500.0,))
     batch = slm.generate_spectra(open_index, cfg)
-    g, hg, st = searcher_open.search(batch, SearchParams(dM=500.0), want_hist=True)
+    g, hg, st = searcher_open.search(batch, SearchParams(dM=500.0, **eng), want_hist=True)
     c, hc, cnt = O.search(open_index, batch, O.default_params(dM=500.0), want_hist=True, want_counters=True)
     compare(g, c, hg, hc, "open")
     check_stats(st, cnt, batch.n_spectra)
     assert (c["flags"] & F.GX_PSM_MODELED).sum() > 250
 
 
-def test_uneven_batch_parity(open_index, searcher_open):
+@pytest.mark.parametrize("eng", ENGINES, ids=ENGINE_IDS)
+def test_uneven_batch_parity(open_index, searcher_open, eng):
     """power-law peak counts up to 2000 (slice tables spill to scratch) and
     per-spectrum dM from {10, 100, 500}"""
-    say("case: uneven batch, power-law peaks 20-2000, dM mix")
+    say(f"case: uneven batch, power-law peaks 20-2000, dM mix, {eng}")
     cfg = slm.SpecConfig(n_spectra=160, seed=33, peak_mode=F.GX_PEAKS_TOTAL_POWER, peaks_min=20,
                          peaks_max=2000, power_alpha=1.2, charge_min=2, charge_max=3, qalen=2048,
                          dm_choices=(10.0, 100.0, 500.0))
     batch = slm.generate_spectra(open_index, cfg)
     assert (np.diff(batch.idx) > 1024 // 5).any()
-    g, hg, st = searcher_open.search(batch, SearchParams(dM=500.0), want_hist=True)
+    g, hg, st = searcher_open.search(batch, SearchParams(dM=500.0, **eng), want_hist=True)
     c, hc, cnt = O.search(open_index, batch, O.default_params(dM=500.0), want_hist=True, want_counters=True)
     compare(g, c, hg, hc, "uneven")
     check_stats(st, cnt, batch.n_spectra)
 
 
-def test_device_resident_path_matches_one_call(small_batch, searcher_small):
-    say("case: device-resident upload/search/download vs one-call search")
-    p = SearchParams(dM=10.0)
+@pytest.mark.parametrize("eng", ENGINES, ids=ENGINE_IDS)
+def test_device_resident_path_matches_one_call(small_batch, searcher_small, eng):
+    say(f"case: device-resident upload/search/download vs one-call search {eng}")
+    p = SearchParams(dM=10.0, **eng)
     g1, _, _ = searcher_small.search(small_batch, p)
     dq = searcher_small.upload(small_batch)
     searcher_small.search_device(dq, p)
@@ -116,23 +126,26 @@ def _edge_batch(index):
                           np.array(prec, np.float32))
 
 
+@pytest.mark.parametrize("eng", ENGINES, ids=ENGINE_IDS)
 @pytest.mark.parametrize("kw", [dict(), dict(min_shp=1), dict(min_shp=0, min_cpsm=0), dict(dM=-1.0),
                                 dict(dM=0.0), dict(min_cpsm=1, expect_max=1e30)])
-def test_edge_cases_parity(small_index, searcher_small, kw):
-    say(f"case: edge batch {kw}")
+def test_edge_cases_parity(small_index, searcher_small, kw, eng):
+    say(f"case: edge batch {kw} {eng}")
     batch = _edge_batch(small_index)
     pd = dict(dM=10.0)
     pd.update(kw)
-    g, hg, _ = searcher_small.search(batch, SearchParams(**pd), want_hist=True)
+    pd_gpu = dict(pd, **eng)
+    g, hg, _ = searcher_small.search(batch, SearchParams(**pd_gpu), want_hist=True)
     c, hc, _ = O.search(small_index, batch, O.default_params(**pd), want_hist=True)
     compare(g, c, hg, hc, f"edge {kw}")
 
 
-def test_empty_batch(searcher_small):
-    say("case: empty batch")
+@pytest.mark.parametrize("eng", ENGINES, ids=ENGINE_IDS)
+def test_empty_batch(searcher_small, eng):
+    say(f"case: empty batch {eng}")
     batch = slm.QueryBatch(np.zeros(1, np.uint32), np.zeros(0, np.int32), np.zeros(0, np.int32),
                            np.zeros(0, np.float32))
-    g, _, st = searcher_small.search(batch, SearchParams())
+    g, _, st = searcher_small.search(batch, SearchParams(**eng))
     assert len(g) == 0 and st["spectra"] == 0
 
 
