@@ -81,6 +81,28 @@ def cpu_baseline(index, batch, params_ffi, budget_s: float):
             "sample": f"first {m} spectra of the same batch, OpenMP over spectra, {dt:.1f} s"}
 
 
+def calibrate_copy(mib: int) -> int:
+    """A plain device-to-device copy of a known size, for scaling the HBM
+    byte counters (FETCH_SIZE / WRITE_SIZE) of rocprofv3 PMC runs."""
+    import ctypes as C
+
+    hip = C.CDLL("libamdhip64.so")
+    n = mib << 20
+    src, dst = C.c_void_p(), C.c_void_p()
+    for p in (src, dst):
+        if hip.hipMalloc(C.byref(p), C.c_size_t(n)) != 0:
+            raise RuntimeError("hipMalloc failed")
+    hip.hipMemset(src, 1, C.c_size_t(n))
+    hip.hipDeviceSynchronize()
+    for _ in range(3):
+        hip.hipMemcpy(dst, src, C.c_size_t(n), 3)  # hipMemcpyDeviceToDevice
+    hip.hipDeviceSynchronize()
+    hip.hipFree(src)
+    hip.hipFree(dst)
+    print(json.dumps({"copy_bytes_per_call": n, "calls": 3}), flush=True)
+    return 0
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -93,7 +115,11 @@ def main() -> int:
     ap.add_argument("--engine", default="batched", choices=["batched", "spectrum"])
     ap.add_argument("--config", type=int, default=-1, help="batched launch configuration (-1: try all, keep the best)")
     ap.add_argument("--compare-plain", action="store_true", help="also time the per-spectrum kernel once")
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
+                    help="only run a device-to-device copy of MIB MiB (HBM counter calibration) and exit")
     args = ap.parse_args()
+    if args.calibrate_copy:
+        return calibrate_copy(args.calibrate_copy)
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for {args.gpus}); the search shards by "
               "spectrum with a replicated index, see DESIGN.md", file=sys.stderr)

@@ -84,7 +84,7 @@ class gx_search_stats(C.Structure):
     _fields_ = [
         ("ions", C.c_uint64), ("bins", C.c_uint64), ("window", C.c_uint64), ("candidates", C.c_uint64),
         ("spectra", C.c_uint64), ("peaks", C.c_uint64), ("fallbacks", C.c_uint64),
-        ("items", C.c_uint64), ("chunks", C.c_uint32), ("config", C.c_int32),
+        ("ions_scanned", C.c_uint64), ("errors", C.c_uint64), ("items", C.c_uint64), ("chunks", C.c_uint32), ("config", C.c_int32),
         ("kernel_ms", C.c_double), ("evalue_ms", C.c_double), ("plan_ms", C.c_double), ("total_ms", C.c_double),
     ]
 

@@ -135,7 +135,7 @@ __global__ void __launch_bounds__(kPlanThreads) plan_spectra_kernel(PlanArgs a) 
             a.pair_item_off[pi] = it;
             a.pair_pos_off[pi] = (uint32_t)pw;
             it += s_tiles[b];
-            if (s_tiles[b]) pw += 2 * nsl;
+            if (s_tiles[b]) pw += nsl * (uint64_t)(s_tiles[b] + 1);
         }
         a.n_legal[s] = (uint32_t)P;
         a.n_items[s] = it;
@@ -219,8 +219,10 @@ __global__ void __launch_bounds__(kPlanThreads) plan_slices_kernel(PlanArgs a) {
     const int nsl = P * nb5;
     const DevBucket bk = a.buckets[b];
     const uint32_t speclen = bk.speclen;
-    const uint32_t lo_id = (uint32_t)a.pair_lo[pi] * speclen;
+    const uint32_t lo = (uint32_t)a.pair_lo[pi];
     const uint32_t end_id = (uint32_t)(a.pair_hi[pi] + 1) * speclen;
+    const uint32_t T = a.pair_pack[pi] * (uint32_t)a.tile_words;  // psids per tile
+    const uint32_t stride = tiles + 1;
     const uint64_t base = (a.pos_off[s] - a.pos_off[a.chunk_s0]) + a.pair_pos_off[pi];
     const int32_t *lq = a.lq + a.qidx[s];
     unsigned long long ions = 0, bins = 0;
@@ -228,14 +230,16 @@ __global__ void __launch_bounds__(kPlanThreads) plan_slices_kernel(PlanArgs a) {
         const int k = j / nb5;
         const uint32_t bin = (uint32_t)(lq[k] - dF + (j - k * nb5));
         const uint32_t st = bk.bA[bin], en = bk.bA[bin + 1];
-        uint32_t lo = en, hi = en;
-        if (en > st) {
-            lo = st + lower_bound_u32(bk.iA + st, en - st, lo_id);
-            hi = lo + lower_bound_u32(bk.iA + lo, en - lo, end_id);
+        uint32_t *out = a.pos + base + (uint64_t)j * stride;
+        // boundary t = first ion of the slice with id >= (lo + t*T)*speclen;
+        // the last one is the window end
+        uint32_t cur = st;
+        for (uint32_t t = 0; t <= tiles; t++) {
+            const uint32_t id = (t == tiles) ? end_id : (lo + t * T) * speclen;
+            if (en > cur) cur += lower_bound_u32(bk.iA + cur, en - cur, id);
+            out[t] = cur;
         }
-        a.pos[base + 2 * (uint64_t)j] = lo;
-        a.pos[base + 2 * (uint64_t)j + 1] = hi;
-        ions += hi - lo;
+        ions += out[tiles] - out[0];
         bins++;
     }
     const uint32_t it0 = a.item_off[s] + a.pair_item_off[pi];
@@ -259,12 +263,13 @@ __global__ void __launch_bounds__(kPlanThreads) plan_slices_kernel(PlanArgs a) {
 // ---------------------------------------------------------------------------
 template <int THREADS, int TILE_WORDS, int SL>
 struct ItemLds {
-    uint32_t cnt[TILE_WORDS];
+    alignas(16) uint32_t cnt[TILE_WORDS];  // count tile; pass-2 accumulators after the scan
     int32_t pk_q[kPeakCap];
     int32_t pk_i[kPeakCap];
-    uint32_t s_a[SL];
-    uint32_t s_n[SL];
-    uint32_t s_off[SL + 1];
+    uint32_t s_a[SL];          // non-empty slices of the tile: first ion
+    uint32_t s_flat[SL + 1];   // exclusive prefix of their lengths
+    uint32_t wsum2[THREADS / 64];
+    uint32_t ne;
     uint32_t cand[kCandLds];
     uint32_t hist[GX_HIST_BINS];
     uint32_t wsum[THREADS / 64];
@@ -272,7 +277,7 @@ struct ItemLds {
     uint32_t w_eq[THREADS / 64];
     uint32_t w_shp[THREADS / 64];
     uint32_t w_cps[THREADS / 64];
-    uint32_t item;
+    uint32_t next[2];          // double-buffered prefetched item index
     uint32_t n_cand;
 };
 
@@ -305,23 +310,45 @@ __device__ void block_excl_scan(uint32_t *v, int n, uint32_t *wsum) {
     __syncthreads();
 }
 
+// merge two top-hit summaries (key, count of candidates equal to its
+// hyperscore, shared ions of the key's hit)
+__device__ __forceinline__ void merge_best(unsigned long long &k, uint32_t &e, uint32_t &shp, unsigned long long k2,
+                                           uint32_t e2, uint32_t shp2) {
+    const uint32_t h1 = (uint32_t)(k >> 32), h2 = (uint32_t)(k2 >> 32);
+    if (h1 == h2) {
+        e += e2;
+        if (k2 > k) {
+            k = k2;
+            shp = shp2;
+        }
+    } else if (h2 > h1) {
+        k = k2;
+        e = e2;
+        shp = shp2;
+    }
+}
+
 template <int THREADS, int TILE_WORDS, int SL>
 __global__ void __launch_bounds__(THREADS) search_items_kernel(ItemArgs a) {
     constexpr int NW = THREADS / 64;
+    constexpr int CB = TILE_WORDS / 3;  // candidates per pass-2 round (3 LDS accumulators each)
     __shared__ ItemLds<THREADS, TILE_WORDS, SL> L;
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
     const int dF = a.dF, nb5 = 2 * dF + 1;
     const uint32_t min_cnt = (uint32_t)max(a.min_shp, 1);
     uint32_t *g_cand = a.cand_scratch + (size_t)blockIdx.x * a.tile_entries_max;
     for (int w = tid; w < TILE_WORDS; w += THREADS) L.cnt[w] = 0;
     for (int w = tid; w < GX_HIST_BINS; w += THREADS) L.hist[w] = 0;
-    unsigned long long my_cand = 0;
+    if (tid == 0) L.next[0] = a.item0 + atomicAdd(a.work_counter, 1u);
+    unsigned long long my_cand = 0, my_scanned = 0, my_errors = 0;
 
-    for (;;) {
-        if (tid == 0) L.item = a.item0 + atomicAdd(a.work_counter, 1u);
+    for (uint32_t par = 0;; par ^= 1u) {
         __syncthreads();
-        const uint32_t item = L.item;
+        const uint32_t item = L.next[par];
         if (item >= a.item1) break;
+        // fetch the next item now; its latency overlaps this one
+        if (tid == 0) L.next[par ^ 1u] = a.item0 + atomicAdd(a.work_counter, 1u);
         const uint32_t s = a.items[2 * (uint64_t)item];
         const uint32_t bt = a.items[2 * (uint64_t)item + 1];
         const uint32_t b = bt & 0xffu, t = bt >> 8;
@@ -338,9 +365,6 @@ __global__ void __launch_bounds__(THREADS) search_items_kernel(ItemArgs a) {
         const uint32_t nent = (uint32_t)(thi - tlo + 1);
         const DevBucket bk = a.buckets[b];
         const uint32_t speclen = bk.speclen;
-        const double inv_d = 1.0 / (double)speclen;
-        const uint32_t tlo_id = (uint32_t)tlo * speclen;
-        const uint32_t thi_id = (uint32_t)(thi + 1) * speclen;
         const int P = (int)a.n_legal[s];
         const uint32_t qoff = a.qidx[s];
         for (int k = tid; k < P; k += THREADS) {
@@ -348,53 +372,92 @@ __global__ void __launch_bounds__(THREADS) search_items_kernel(ItemArgs a) {
             L.pk_i[k] = a.li[qoff + k];
         }
         const int nsl = P * nb5;
-        const uint64_t pbase = (a.pos_off[s] - a.pos_off[a.chunk_s0]) + a.pair_pos_off[pi];
+        const uint32_t stride = tiles + 1;
+        const uint32_t *pos = a.pos + (a.pos_off[s] - a.pos_off[a.chunk_s0]) + a.pair_pos_off[pi];
         const uint32_t *ia = bk.iA;
         // ---- pass 1: shared-peak counts of every entry in the tile ----
+        const uint32_t dm = bk.div_m, dsh = bk.div_s;
         for (int c0 = 0; c0 < nsl; c0 += SL) {
             const int ns = min(SL, nsl - c0);
-            for (int j = tid; j < ns; j += THREADS) {
-                const uint32_t a0 = a.pos[pbase + 2 * (uint64_t)(c0 + j)];
-                const uint32_t z0 = a.pos[pbase + 2 * (uint64_t)(c0 + j) + 1];
-                uint32_t at = a0, zt = z0;
-                if (tiles > 1 && z0 > a0) {
-                    if (t > 0) at = a0 + lower_bound_u32(ia + a0, z0 - a0, tlo_id);
-                    if (t + 1 < tiles) zt = a0 + lower_bound_u32(ia + a0, z0 - a0, thi_id);
-                }
-                L.s_a[j] = at;
-                L.s_n[j] = zt - at;
-                L.s_off[j] = (zt - at + 63) >> 6;
+            // slice ranges of this tile; non-empty ones compacted in order
+            static_assert(SL == THREADS, "one slice per thread");
+            uint32_t at = 0, len = 0;
+            if (tid < ns) {
+                const uint32_t *pj = pos + (uint64_t)(c0 + tid) * stride + t;
+                at = pj[0];
+                len = pj[1] - at;
+            }
+            const uint32_t fl = len ? 1u : 0u;
+            const uint32_t il = wave_incl_scan(len, lane), ifl = wave_incl_scan(fl, lane);
+            if (lane == 63) {
+                L.wsum[wave] = il;
+                L.wsum2[wave] = ifl;
             }
             __syncthreads();
-            block_excl_scan<THREADS, SL>(L.s_off, ns, L.wsum);
-            const uint32_t np = L.s_off[ns];
-            const uint32_t per = (np + NW - 1) / NW;
-            const uint32_t q0 = min(np, (uint32_t)wave * per), q1 = min(np, q0 + per);
-            if (q0 < q1) {
-                int j = upper_bound_lds(L.s_off, ns + 1, q0) - 1;
-                for (uint32_t q = q0; q < q1; q += kUnroll) {
-                    uint32_t raw[kUnroll];
-                    bool ok[kUnroll];
+            uint32_t bl = 0, bf = 0;
+            for (int w = 0; w < wave; w++) {
+                bl += L.wsum[w];
+                bf += L.wsum2[w];
+            }
+            if (fl) {
+                L.s_a[bf + ifl - 1] = at;
+                L.s_flat[bf + ifl - 1] = bl + il - len;
+            }
+            if (tid == THREADS - 1) {
+                L.s_flat[bf + ifl] = bl + il;
+                L.ne = bf + ifl;
+            }
+            __syncthreads();
+            // the tile's ions, flattened over the non-empty slices; each wave
+            // walks a contiguous range of 64-id chunks, so the slice of a
+            // chunk's first lane carries over from the previous chunk
+            const uint32_t ne = L.ne;
+            const uint32_t N = L.s_flat[ne];
+            const uint32_t nch = (N + 63) >> 6;
+            const uint32_t per = (nch + NW - 1) / NW;
+            uint32_t c = min(nch, (uint32_t)wave * per);
+            const uint32_t cend = min(nch, c + per);
+            if (c < cend) {
+                uint32_t j0 = (uint32_t)upper_bound_lds(L.s_flat, (int)ne + 1, c << 6) - 1;
+                constexpr int U = 2 * kUnroll;
+                for (; c < cend; c += U) {
+                    uint32_t raw[U];
+                    bool ok[U];
 #pragma unroll
-                    for (int u = 0; u < kUnroll; u++) {
-                        const uint32_t qq = q + u;
-                        ok[u] = false;
+                    for (int u = 0; u < U; u++) {
+                        const uint32_t f0 = (c + u) << 6;
+                        const uint32_t f = f0 + (uint32_t)lane;
+                        ok[u] = (c + u < cend) && f < N;
                         raw[u] = 0;
-                        if (qq < q1) {
-                            while (L.s_off[j + 1] <= qq) j++;
-                            const uint32_t k = (qq - L.s_off[j]) << 6;
-                            const uint32_t pos = k + (uint32_t)lane;
-                            if (pos < L.s_n[j]) {
-                                ok[u] = true;
-                                raw[u] = ia[L.s_a[j] + pos];
+                        if (c + u < cend) {
+                            // lane -> slice: the slices after j0 that start inside
+                            // the chunk set bit (start - f0) of a wave-wide mask;
+                            // lane L's slice is j0 + the starts at positions <= L
+                            const uint32_t k = j0 + 1 + (uint32_t)lane;
+                            unsigned long long bit = 0;
+                            if (k < ne) {
+                                const uint32_t d = L.s_flat[k] - f0;
+                                if (d < 64) bit = 1ull << d;
                             }
+#pragma unroll
+                            for (int o = 1; o < 64; o <<= 1) bit |= __shfl_xor(bit, o, 64);
+                            const unsigned long long upto =
+                                lane == 63 ? ~0ull : ((1ull << (uint32_t)(lane + 1)) - 1ull);
+                            const uint32_t j = j0 + (uint32_t)__popcll(bit & upto);
+                            j0 += (uint32_t)__popcll(bit);
+                            if (ok[u]) raw[u] = ia[L.s_a[j] + (f - L.s_flat[j])];
                         }
                     }
 #pragma unroll
-                    for (int u = 0; u < kUnroll; u++) {
+                    for (int u = 0; u < U; u++) {
                         if (ok[u]) {
-                            const uint32_t el = div_entry(raw[u], inv_d) - (uint32_t)tlo;
-                            atomicAdd(&L.cnt[el >> lp], 1u << ((el & (pack - 1)) * fbits));
+                            const uint32_t el = (__umulhi(raw[u], dm) >> dsh) - (uint32_t)tlo;
+                            if (el < nent) {
+                                atomicAdd(&L.cnt[el >> lp], 1u << ((el & (pack - 1)) * fbits));
+                                my_scanned++;
+                            } else {
+                                my_errors++;  // never expected: an id outside the tile
+                            }
                         }
                     }
                 }
@@ -404,56 +467,81 @@ __global__ void __launch_bounds__(THREADS) search_items_kernel(ItemArgs a) {
         // ---- candidates: entries with count >= min_shp; clear the tile ----
         if (tid == 0) L.n_cand = 0;
         __syncthreads();
-        const uint32_t nw = (nent + pack - 1) >> lp;
-        for (uint32_t w = tid; w < nw; w += THREADS) {
-            const uint32_t v = L.cnt[w];
-            if (v == 0) continue;
-            L.cnt[w] = 0;
-            for (uint32_t e = 0; e < pack; e++) {
-                const uint32_t c = (v >> (e * fbits)) & fmask;
-                const uint32_t el = (w << lp) + e;
-                if (c >= min_cnt && el < nent) {
-                    const uint32_t pos = atomicAdd(&L.n_cand, 1u);
-                    const uint32_t psid = (uint32_t)tlo + el;
-                    if (pos < (uint32_t)kCandLds)
-                        L.cand[pos] = psid;
-                    else
-                        g_cand[pos - kCandLds] = psid;
+        {
+            // SWAR "some counter >= min_cnt" test per word (min_cnt <= 128)
+            const uint32_t ones = pack == 4 ? 0x01010101u : (pack == 2 ? 0x00010001u : 1u);
+            const uint32_t hib = ones << (fbits - 1);
+            const uint32_t low = pack == 1 ? 0x7fffffffu : (~hib);
+            const bool swar = min_cnt <= 128;
+            const uint32_t addv = ((1u << (fbits - 1)) - min_cnt) * ones;
+            const uint32_t nw = (nent + pack - 1) >> lp;
+            const uint32_t nq = (nw + 3) >> 2;
+            uint4 *cq = reinterpret_cast<uint4 *>(L.cnt);
+            for (uint32_t q = tid; q < nq; q += THREADS) {
+                const uint4 v4 = cq[q];
+                if ((v4.x | v4.y | v4.z | v4.w) == 0) continue;
+                cq[q] = make_uint4(0, 0, 0, 0);
+                const uint32_t vv[4] = {v4.x, v4.y, v4.z, v4.w};
+#pragma unroll
+                for (int i = 0; i < 4; i++) {
+                    const uint32_t v = vv[i];
+                    if (v == 0) continue;
+                    if (swar && ((((v & low) + addv) | v) & hib) == 0) continue;
+                    const uint32_t w = 4 * q + i;
+                    for (uint32_t e = 0; e < pack; e++) {
+                        const uint32_t cc = (v >> (e * fbits)) & fmask;
+                        const uint32_t el = (w << lp) + e;
+                        if (cc >= min_cnt && el < nent) {
+                            const uint32_t p = atomicAdd(&L.n_cand, 1u);
+                            const uint32_t psid = (uint32_t)tlo + el;
+                            if (p < (uint32_t)kCandLds)
+                                L.cand[p] = psid;
+                            else
+                                g_cand[p - kCandLds] = psid;
+                        }
+                    }
                 }
             }
         }
         __syncthreads();
-        // ---- pass 2: exact bc/yc/ibc/iyc per candidate, one wave each ----
+        // ---- pass 2: exact bc/yc/ibc/iyc of every candidate, flattened over
+        // (candidate, fragment ion) and summed with LDS atomics ----
         const uint32_t ncand = L.n_cand;
         const uint32_t half = speclen >> 1;
-        float bh = 0.0f;
-        int bb = 0x7fffffff, bp = 0x7fffffff;
-        uint32_t bshp = 0, beq = 0, bcps = 0;
-        for (uint32_t ci = wave; ci < ncand; ci += NW) {
-            const uint32_t psid = ci < (uint32_t)kCandLds ? L.cand[ci] : g_cand[ci - kCandLds];
-            const uint32_t *bins = bk.ion_bin + (size_t)psid * speclen;
-            uint32_t bc = 0, yc = 0, ibc = 0, iyc = 0;
-            for (uint32_t r = lane; r < speclen; r += 64) {
-                const int32_t x = (int32_t)bins[r];
+        unsigned long long bkey = 0;
+        uint32_t beq = 0, bshp = 0, bcps = 0;
+        uint32_t *acc_n = L.cnt, *acc_ib = L.cnt + CB, *acc_iy = L.cnt + 2 * CB;
+        for (uint32_t c0 = 0; c0 < ncand; c0 += CB) {
+            const uint32_t nc = min((uint32_t)CB, ncand - c0);
+            const uint32_t tasks = nc * speclen;
+            for (uint32_t tk = tid; tk < tasks; tk += THREADS) {
+                const uint32_t c = __umulhi(tk, bk.div_m) >> bk.div_s;
+                const uint32_t r = tk - c * speclen;
+                const uint32_t ci = c0 + c;
+                const uint32_t psid = ci < (uint32_t)kCandLds ? L.cand[ci] : g_cand[ci - kCandLds];
+                const int32_t x = (int32_t)bk.ion_bin[(size_t)psid * speclen + r];
                 int k = lower_bound_lds(L.pk_q, P, x - dF);
                 uint32_t n = 0, sum = 0;
                 for (; k < P && L.pk_q[k] <= x + dF; k++) {
                     n++;
                     sum += (uint32_t)L.pk_i[k];
                 }
-                if (r < half) {
-                    bc += n;
-                    ibc += sum;
-                } else {
-                    yc += n;
-                    iyc += sum;
+                if (n) {
+                    const bool isy = r >= half;
+                    atomicAdd(&acc_n[c], isy ? (n << 16) : n);
+                    atomicAdd(isy ? &acc_iy[c] : &acc_ib[c], sum);
                 }
             }
-            bc = wave_sum_u32(bc);
-            yc = wave_sum_u32(yc);
-            ibc = wave_sum_u32(ibc);
-            iyc = wave_sum_u32(iyc);
-            if (lane == 0) {
+            __syncthreads();
+            for (uint32_t c = tid; c < nc; c += THREADS) {
+                const uint32_t v = acc_n[c];
+                const uint32_t bc = v & 0xffffu, yc = v >> 16;
+                const uint32_t ibc = acc_ib[c], iyc = acc_iy[c];
+                acc_n[c] = 0;
+                acc_ib[c] = 0;
+                acc_iy[c] = 0;
+                const uint32_t ci = c0 + c;
+                const uint32_t psid = ci < (uint32_t)kCandLds ? L.cand[ci] : g_cand[ci - kCandLds];
                 const uint32_t shpk = bc + yc;
                 if (shpk >= (uint32_t)a.min_shp && (int)bc < a.lgfact_n && (int)yc < a.lgfact_n) {
                     float hs = hyperscore(bc, yc, ibc, iyc, a.lgfact);
@@ -461,26 +549,27 @@ __global__ void __launch_bounds__(THREADS) search_items_kernel(ItemArgs a) {
                         if (hs >= 80.0f) hs = 80.0f - 1;
                         bcps++;
                         atomicAdd(&L.hist[hyp_bin(hs)], 1u);
-                        if (better(hs, (int)b, (int)psid, bh, bb, bp)) {
-                            if (hs != bh) beq = 0;
-                            bh = hs;
-                            bb = (int)b;
-                            bp = (int)psid;
-                            bshp = shpk;
-                        }
-                        if (hs == bh) beq++;
+                        merge_best(bkey, beq, bshp, hit_key(hs, b, psid), 1u, shpk);
                     }
                 }
             }
+            __syncthreads();
         }
+        // ---- block reduction of the top hit, flush to the spectrum ----
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) {
+            const unsigned long long k2 = __shfl_xor(bkey, o, 64);
+            const uint32_t e2 = __shfl_xor(beq, o, 64), s2 = __shfl_xor(bshp, o, 64);
+            merge_best(bkey, beq, bshp, k2, e2, s2);
+        }
+        bcps = wave_sum_u32(bcps);
         if (lane == 0) {
-            L.w_key[wave] = bcps ? hit_key(bh, (uint32_t)bb, (uint32_t)bp) : 0ull;
+            L.w_key[wave] = bkey;
             L.w_eq[wave] = beq;
             L.w_shp[wave] = bshp;
             L.w_cps[wave] = bcps;
         }
         __syncthreads();
-        // ---- flush to the spectrum's accumulators ----
         for (int w = tid; w < GX_HIST_BINS; w += THREADS) {
             const uint32_t v = L.hist[w];
             if (v) {
@@ -493,22 +582,22 @@ __global__ void __launch_bounds__(THREADS) search_items_kernel(ItemArgs a) {
             uint32_t eq = 0, shp = 0, cps = 0;
             for (int w = 0; w < NW; w++) {
                 cps += L.w_cps[w];
-                if (L.w_key[w] > key) {
-                    key = L.w_key[w];
-                    shp = L.w_shp[w];
-                }
+                merge_best(key, eq, shp, L.w_key[w], L.w_eq[w], L.w_shp[w]);
             }
-            for (int w = 0; w < NW; w++)
-                if (L.w_key[w] && (L.w_key[w] >> 32) == (key >> 32)) eq += L.w_eq[w];
             a.item_key[item] = key;
             a.item_eq[item] = eq;
             a.item_shp[item] = shp;
             if (cps) atomicAdd(&a.cpsms[s], cps);
             my_cand += cps;
         }
-        __syncthreads();
     }
     if (tid == 0) stat_add(a.stats, ST_CAND, my_cand);
+    my_scanned = wave_sum_u64(my_scanned);
+    my_errors = wave_sum_u64(my_errors);
+    if (lane == 0) {
+        stat_add(a.stats, ST_SCANNED, my_scanned);
+        stat_add(a.stats, ST_ERRORS, my_errors);
+    }
 }
 
 // ---------------------------------------------------------------------------

@@ -123,6 +123,32 @@ int ensure_scratch(gx_ctx *c, int n_blocks, uint32_t max_slices) {
     return GX_SUCCESS;
 }
 
+// Division by an invariant integer d (2 <= d < 2^12) for every n < 2^31 as a
+// multiply-high and a shift (Granlund & Montgomery, PLDI 1994, Thm 4.2): with
+// l = ceil(log2 d), k = 31 + l and m = ceil(2^k / d) (< 2^32),
+// floor(n / d) = floor(n * m / 2^k) = umulhi(n, m) >> (k - 32).
+// Checked here against exact division at the values most likely to break it.
+bool make_divisor(uint32_t d, uint32_t &m, uint32_t &s) {
+    if (d < 2 || d >= 4096) return false;
+    uint32_t l = 0;
+    while ((1u << l) < d) l++;
+    const uint32_t k = 31 + l;
+    const unsigned __int128 two_k = (unsigned __int128)1 << k;
+    const unsigned __int128 mm = (two_k + d - 1) / d;
+    if (mm >= ((unsigned __int128)1 << 32)) return false;
+    m = (uint32_t)mm;
+    s = k - 32;
+    auto div = [&](uint64_t n) { return (uint32_t)(((n * (uint64_t)m) >> 32) >> s); };
+    const uint64_t top = (1ull << 31) - 1;
+    for (uint64_t q = 0; q < 4096; q++) {
+        for (uint64_t n : {q * d, q * d + d - 1, top - q * d, (top / d - q) * d, (top / d - q) * d + d - 1}) {
+            if (n > top) continue;
+            if (div(n) != n / d) return false;
+        }
+    }
+    return true;
+}
+
 int check_params(const gx_search_params *p, const gx_dindex *idx) {
     if (p->dF < 0 || p->dF > 64 || p->scale < 1 || p->max_mass < 1 || p->min_shp < 0 || p->min_cpsm < 0)
         return GX_ERR_INVLD_PARAM;
@@ -225,6 +251,7 @@ extern "C" int gx_index_upload(gx_ctx *c, const gx_bucket *b, int nb, int maxz, 
         if ((uint64_t)x.n_entries * x.speclen != x.n_ions || x.n_ions >= (1ull << 32)) return fail(GX_ERR_INVLD_SIZE);
         if (x.bA[x.nbins] != x.n_ions) return fail(GX_ERR_INVLD_INDEX);
         if (x.n_entries >= (1u << 26)) return fail(GX_ERR_INVLD_SIZE);  // packed top-hit key holds 26-bit psids
+        if (x.n_ions >= (1ull << 31)) return fail(GX_ERR_INVLD_SIZE);     // ion ids use the 31-bit divisor
     }
     d->nbins = nbins;
     for (int i = 0; i < nb; i++) {
@@ -232,6 +259,7 @@ extern "C" int gx_index_upload(gx_ctx *c, const gx_bucket *b, int nb, int maxz, 
         DevBucket db{};
         db.n_entries = x.n_entries;
         db.speclen = x.speclen;
+        if (x.n_entries > 0 && !make_divisor(x.speclen, db.div_m, db.div_s)) return fail(GX_ERR_INVLD_INDEX);
         if (x.n_entries > 0) {
             float *m;
             uint32_t *bA, *iA, *ib;
@@ -353,6 +381,8 @@ void fill_stats(gx_search_stats *stats, const unsigned long long *h) {
     stats->spectra = h[ST_SPECTRA];
     stats->peaks = h[ST_PEAKS];
     stats->fallbacks = h[ST_FALLBACK];
+    stats->ions_scanned = h[ST_SCANNED];
+    stats->errors = h[ST_ERRORS];
 }
 
 float elapsed(hipEvent_t a, hipEvent_t b) {

@@ -406,6 +406,7 @@ __global__ void __launch_bounds__(kSearchThreads, 4) search_kernel(SearchArgs a)
         __syncthreads();
     }
     atomicAdd(&L.st[ST_IONS], my_ions);
+    atomicAdd(&L.st[ST_SCANNED], my_ions);
     atomicAdd(&L.st[ST_BINS], my_bins);
     __syncthreads();
     if (tid < ST_N && L.st[tid]) atomicAdd(&a.stats[tid], L.st[tid]);

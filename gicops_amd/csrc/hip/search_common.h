@@ -18,10 +18,13 @@ struct DevBucket {
     const uint32_t *ion_bin;
     uint32_t n_entries;
     uint32_t speclen;
+    // exact division by speclen for n < 2^31: n / speclen == umulhi(n, div_m) >> div_s
+    uint32_t div_m;
+    uint32_t div_s;
 };
 
 // Work counters accumulated by the search kernel (u64 each).
-enum { ST_IONS = 0, ST_BINS, ST_WINDOW, ST_CAND, ST_PEAKS, ST_SPECTRA, ST_FALLBACK, ST_N };
+enum { ST_IONS = 0, ST_BINS, ST_WINDOW, ST_CAND, ST_PEAKS, ST_SPECTRA, ST_FALLBACK, ST_SCANNED, ST_ERRORS, ST_N };
 
 struct SearchArgs {
     const DevBucket *buckets;

@@ -51,6 +51,8 @@ ENGINE_IDS = ["batched-c0", "batched-c1", "batched-c2", "spectrum"]
 
 def check_stats(st, cnt, n):
     assert st["spectra"] == n
+    assert st["errors"] == 0
+    assert st["ions_scanned"] == st["ions"], (st["ions_scanned"], st["ions"])
     assert st["ions"] == int(cnt["ions"].sum())
     assert st["bins"] == int(cnt["bins"].sum())
     assert st["window"] == int(cnt["window"].sum())
@@ -63,8 +65,8 @@ def test_small_closed_search_parity(small_index, small_batch, searcher_small, en
     g, hg, st = searcher_small.search(small_batch, p, want_hist=True)
     c, hc, cnt = O.search(small_index, small_batch, O.default_params(dM=10.0), want_hist=True,
                           want_counters=True)
-    compare(g, c, hg, hc, "small")
     check_stats(st, cnt, small_batch.n_spectra)
+    compare(g, c, hg, hc, "small")
     assert st["candidates"] == int(c["cpsms"].sum())
 
 
@@ -76,8 +78,8 @@ def test_open_search_parity(open_index, searcher_open, eng):
     batch = slm.generate_spectra(open_index, cfg)
     g, hg, st = searcher_open.search(batch, SearchParams(dM=500.0, **eng), want_hist=True)
     c, hc, cnt = O.search(open_index, batch, O.default_params(dM=500.0), want_hist=True, want_counters=True)
-    compare(g, c, hg, hc, "open")
     check_stats(st, cnt, batch.n_spectra)
+    compare(g, c, hg, hc, "open")
     assert (c["flags"] & F.GX_PSM_MODELED).sum() > 250
 
 
@@ -93,8 +95,8 @@ def test_uneven_batch_parity(open_index, searcher_open, eng):
     assert (np.diff(batch.idx) > 1024 // 5).any()
     g, hg, st = searcher_open.search(batch, SearchParams(dM=500.0, **eng), want_hist=True)
     c, hc, cnt = O.search(open_index, batch, O.default_params(dM=500.0), want_hist=True, want_counters=True)
-    compare(g, c, hg, hc, "uneven")
     check_stats(st, cnt, batch.n_spectra)
+    compare(g, c, hg, hc, "uneven")
 
 
 @pytest.mark.parametrize("eng", ENGINES, ids=ENGINE_IDS)
