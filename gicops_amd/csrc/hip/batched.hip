@@ -29,6 +29,8 @@ constexpr int kPlanThreads = 256;
 constexpr int kPeakCap = kMaxPeaksLds;  // 2048 legal peaks per spectrum
 constexpr int kStatSpread = kStatRows;
 constexpr int kUnroll = 4;              // pieces in flight per wave
+constexpr uint32_t kWholeBin = 64;      // bins this small skip the boundary search
+constexpr uint32_t kWholeFlag = 0x80000000u;  // marks a whole-bin slice start (ion positions < 2^31)
 
 __device__ __forceinline__ void stat_add(unsigned long long *stats, int which, unsigned long long v) {
     if (v) atomicAdd(&stats[(blockIdx.x % kStatSpread) * ST_N + which], v);
@@ -225,21 +227,28 @@ __global__ void __launch_bounds__(kPlanThreads) plan_slices_kernel(PlanArgs a) {
     const uint32_t stride = tiles + 1;
     const uint64_t base = (a.pos_off[s] - a.pos_off[a.chunk_s0]) + a.pair_pos_off[pi];
     const int32_t *lq = a.lq + a.qidx[s];
-    unsigned long long ions = 0, bins = 0;
+    unsigned long long bins = 0;
     for (int j = tid; j < nsl; j += kPlanThreads) {
         const int k = j / nb5;
         const uint32_t bin = (uint32_t)(lq[k] - dF + (j - k * nb5));
         const uint32_t st = bk.bA[bin], en = bk.bA[bin + 1];
         uint32_t *out = a.pos + base + (uint64_t)j * stride;
-        // boundary t = first ion of the slice with id >= (lo + t*T)*speclen;
-        // the last one is the window end
+        // bins of at most kWholeBin ions are streamed whole by the search
+        // kernel (flagged start, every tile filters by id); larger ones get
+        // boundary t = first ion of the slice with id >= (lo + t*T)*speclen,
+        // the last one being the window end
+        if (en - st <= kWholeBin) {
+            out[0] = st | kWholeFlag;
+            out[1] = en;
+            bins++;
+            continue;
+        }
         uint32_t cur = st;
         for (uint32_t t = 0; t <= tiles; t++) {
             const uint32_t id = (t == tiles) ? end_id : (lo + t * T) * speclen;
             if (en > cur) cur += lower_bound_u32(bk.iA + cur, en - cur, id);
             out[t] = cur;
         }
-        ions += out[tiles] - out[0];
         bins++;
     }
     const uint32_t it0 = a.item_off[s] + a.pair_item_off[pi];
@@ -247,17 +256,10 @@ __global__ void __launch_bounds__(kPlanThreads) plan_slices_kernel(PlanArgs a) {
         a.items[2 * (uint64_t)(it0 + t)] = s;
         a.items[2 * (uint64_t)(it0 + t) + 1] = b | (t << 8);
     }
-    ions = wave_sum_u64(ions);
     bins = wave_sum_u64(bins);
-    if (lane == 0) {
-        atomicAdd(&s_st[0], ions);
-        atomicAdd(&s_st[1], bins);
-    }
+    if (lane == 0) atomicAdd(&s_st[1], bins);
     __syncthreads();
-    if (tid == 0) {
-        stat_add(a.stats, ST_IONS, s_st[0]);
-        stat_add(a.stats, ST_BINS, s_st[1]);
-    }
+    if (tid == 0) stat_add(a.stats, ST_BINS, s_st[1]);
 }
 
 // ---------------------------------------------------------------------------
@@ -269,6 +271,7 @@ struct ItemLds {
     uint32_t s_a[SL];          // non-empty slices of the tile: first ion
     uint32_t s_flat[SL + 1];   // exclusive prefix of their lengths
     uint32_t wsum2[THREADS / 64];
+    uint16_t cj0[THREADS];     // slice of each chunk's first id, per round
     uint32_t ne;
     uint32_t cand[kCandLds];
     uint32_t hist[GX_HIST_BINS];
@@ -365,6 +368,8 @@ __global__ void __launch_bounds__(THREADS) search_items_kernel(ItemArgs a) {
         const uint32_t nent = (uint32_t)(thi - tlo + 1);
         const DevBucket bk = a.buckets[b];
         const uint32_t speclen = bk.speclen;
+        const uint32_t tlo_id = (uint32_t)tlo * speclen;
+        const uint32_t thi_id = (uint32_t)(thi + 1) * speclen;
         const int P = (int)a.n_legal[s];
         const uint32_t qoff = a.qidx[s];
         for (int k = tid; k < P; k += THREADS) {
@@ -383,9 +388,15 @@ __global__ void __launch_bounds__(THREADS) search_items_kernel(ItemArgs a) {
             static_assert(SL == THREADS, "one slice per thread");
             uint32_t at = 0, len = 0;
             if (tid < ns) {
-                const uint32_t *pj = pos + (uint64_t)(c0 + tid) * stride + t;
-                at = pj[0];
-                len = pj[1] - at;
+                const uint32_t *ps = pos + (uint64_t)(c0 + tid) * stride;
+                const uint32_t p0 = ps[0];
+                if (p0 & kWholeFlag) {  // small bin: the whole bin, filtered by id below
+                    at = p0 & ~kWholeFlag;
+                    len = ps[1] - at;
+                } else {
+                    at = ps[t];
+                    len = ps[t + 1] - at;
+                }
             }
             const uint32_t fl = len ? 1u : 0u;
             const uint32_t il = wave_incl_scan(len, lane), ifl = wave_incl_scan(fl, lane);
@@ -408,49 +419,55 @@ __global__ void __launch_bounds__(THREADS) search_items_kernel(ItemArgs a) {
                 L.ne = bf + ifl;
             }
             __syncthreads();
-            // the tile's ions, flattened over the non-empty slices; each wave
-            // walks a contiguous range of 64-id chunks, so the slice of a
-            // chunk's first lane carries over from the previous chunk
+            // the tile's ions, flattened over the non-empty slices, in rounds
+            // of THREADS 64-id chunks. The slice holding each chunk's first id
+            // is found first (one thread per chunk); within a chunk, the slices
+            // that start inside it form a bit mask built with scalar readlanes,
+            // and lane L's slice is the chunk's first plus the starts <= L.
             const uint32_t ne = L.ne;
             const uint32_t N = L.s_flat[ne];
             const uint32_t nch = (N + 63) >> 6;
-            const uint32_t per = (nch + NW - 1) / NW;
-            uint32_t c = min(nch, (uint32_t)wave * per);
-            const uint32_t cend = min(nch, c + per);
-            if (c < cend) {
-                uint32_t j0 = (uint32_t)upper_bound_lds(L.s_flat, (int)ne + 1, c << 6) - 1;
+            for (uint32_t r0 = 0; r0 < nch; r0 += THREADS) {
+                const uint32_t nr = min((uint32_t)THREADS, nch - r0);
+                if ((uint32_t)tid < nr)
+                    L.cj0[tid] = (uint16_t)(upper_bound_lds(L.s_flat, (int)ne + 1, (r0 + tid) << 6) - 1);
+                __syncthreads();
                 constexpr int U = 2 * kUnroll;
-                for (; c < cend; c += U) {
+                for (uint32_t c = (uint32_t)wave * U; c < nr; c += NW * U) {
                     uint32_t raw[U];
                     bool ok[U];
 #pragma unroll
                     for (int u = 0; u < U; u++) {
-                        const uint32_t f0 = (c + u) << 6;
-                        const uint32_t f = f0 + (uint32_t)lane;
-                        ok[u] = (c + u < cend) && f < N;
+                        ok[u] = false;
                         raw[u] = 0;
-                        if (c + u < cend) {
-                            // lane -> slice: the slices after j0 that start inside
-                            // the chunk set bit (start - f0) of a wave-wide mask;
-                            // lane L's slice is j0 + the starts at positions <= L
+                        const uint32_t cc = c + u;
+                        if (cc < nr) {
+                            const uint32_t f0 = (r0 + cc) << 6;
+                            const uint32_t f = f0 + (uint32_t)lane;
+                            const uint32_t j0 = L.cj0[cc];
                             const uint32_t k = j0 + 1 + (uint32_t)lane;
-                            unsigned long long bit = 0;
-                            if (k < ne) {
-                                const uint32_t d = L.s_flat[k] - f0;
-                                if (d < 64) bit = 1ull << d;
+                            uint32_t d = 64;
+                            if (k < ne) d = L.s_flat[k] - f0;
+                            const unsigned long long inside = __ballot(d < 64);
+                            uint32_t j = j0;
+                            if (inside) {
+                                unsigned long long starts = 0;
+                                const int m = __popcll(inside);  // slices j0+1 .. j0+m start here
+                                for (int i = 0; i < m; i++)
+                                    starts |= 1ull << __builtin_amdgcn_readlane(d, i);
+                                const unsigned long long upto =
+                                    lane == 63 ? ~0ull : ((1ull << (uint32_t)(lane + 1)) - 1ull);
+                                j = j0 + (uint32_t)__popcll(starts & upto);
                             }
-#pragma unroll
-                            for (int o = 1; o < 64; o <<= 1) bit |= __shfl_xor(bit, o, 64);
-                            const unsigned long long upto =
-                                lane == 63 ? ~0ull : ((1ull << (uint32_t)(lane + 1)) - 1ull);
-                            const uint32_t j = j0 + (uint32_t)__popcll(bit & upto);
-                            j0 += (uint32_t)__popcll(bit);
-                            if (ok[u]) raw[u] = ia[L.s_a[j] + (f - L.s_flat[j])];
+                            if (f < N) {
+                                ok[u] = true;
+                                raw[u] = ia[L.s_a[j] + (f - L.s_flat[j])];
+                            }
                         }
                     }
 #pragma unroll
                     for (int u = 0; u < U; u++) {
-                        if (ok[u]) {
+                        if (ok[u] && raw[u] >= tlo_id && raw[u] < thi_id) {
                             const uint32_t el = (__umulhi(raw[u], dm) >> dsh) - (uint32_t)tlo;
                             if (el < nent) {
                                 atomicAdd(&L.cnt[el >> lp], 1u << ((el & (pack - 1)) * fbits));
@@ -461,6 +478,7 @@ __global__ void __launch_bounds__(THREADS) search_items_kernel(ItemArgs a) {
                         }
                     }
                 }
+                __syncthreads();
             }
             __syncthreads();
         }
@@ -595,6 +613,8 @@ __global__ void __launch_bounds__(THREADS) search_items_kernel(ItemArgs a) {
     my_scanned = wave_sum_u64(my_scanned);
     my_errors = wave_sum_u64(my_errors);
     if (lane == 0) {
+        // H (window-restricted ion ids) is counted where they are consumed
+        stat_add(a.stats, ST_IONS, my_scanned);
         stat_add(a.stats, ST_SCANNED, my_scanned);
         stat_add(a.stats, ST_ERRORS, my_errors);
     }
