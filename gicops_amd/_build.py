@@ -66,6 +66,23 @@ def build_hip() -> Path:
     return out
 
 
+def build_adapter() -> Path:
+    """C++ host side with the reference's types above the C ABI, and the
+    end-to-end driver the GPU tests run."""
+    LIBDIR.mkdir(parents=True, exist_ok=True)
+    out = LIBDIR / "libgicops_adapter.so"
+    tmp = out.with_suffix(".so.tmp")
+    adir = CSRC / "adapter"
+    _run(["g++", "-std=c++17", "-O2", "-fPIC", "-shared", "-ffp-contract=off", "-Wall", "-Wextra",
+          f"-I{INCLUDE}", f"-I{adir}", adir / "dslim_amd.cpp", "-o", tmp,
+          f"-L{LIBDIR}", "-lgicops_hip", "-Wl,-rpath,$ORIGIN"])
+    os.replace(tmp, out)
+    exe = LIBDIR / "adapter_e2e"
+    _run(["g++", "-std=c++17", "-O2", "-Wall", f"-I{INCLUDE}", f"-I{adir}", ROOT / "tests" / "cpp" / "adapter_e2e.cpp",
+          "-o", exe, f"-L{LIBDIR}", "-lgicops_adapter", "-lgicops_host", "-lgicops_hip", "-Wl,-rpath,$ORIGIN"])
+    return out
+
+
 def build_oracle() -> Path:
     _run(["make", "-C", str(ORACLE), "-B"])
     return ORACLE / "liboracle.so"
@@ -75,9 +92,11 @@ def build_all() -> None:
     build_host()
     build_oracle()
     build_hip()
+    build_adapter()
 
 
 if __name__ == "__main__":
     what = sys.argv[1:] or ["all"]
     for w in what:
-        {"host": build_host, "hip": build_hip, "oracle": build_oracle, "all": build_all}[w]()
+        {"host": build_host, "hip": build_hip, "oracle": build_oracle, "adapter": build_adapter,
+         "all": build_all}[w]()
