@@ -51,7 +51,12 @@ class GxError(RuntimeError):
 
 def check(func: str, code: int) -> None:
     if code != GX_SUCCESS:
-        raise GxError(func, code)
+        err = GxError(func, code)
+        if code == GX_ERR_DEVICE and _HIP is not None:
+            buf = C.create_string_buffer(512)
+            _HIP.gx_last_error(buf, 512)
+            err = GxError(f"{func} [{buf.value.decode(errors='replace')}]", code)
+        raise err
 
 
 # ---- structs ----------------------------------------------------------------
@@ -76,7 +81,7 @@ class gx_search_params(C.Structure):
         ("dF", C.c_int32), ("scale", C.c_int32), ("max_mass", C.c_int32), ("min_shp", C.c_int32),
         ("min_cpsm", C.c_int32), ("evalue_mode", C.c_int32), ("dM", C.c_double),
         ("expect_max", C.c_double), ("surrogate_conf", C.c_float), ("engine", C.c_int32),
-        ("config", C.c_int32), ("use_graph", C.c_int32),
+        ("config", C.c_int32), ("use_graph", C.c_int32), ("split", C.c_int32), ("reserved", C.c_int32),
     ]
 
 
@@ -86,6 +91,17 @@ class gx_search_stats(C.Structure):
         ("spectra", C.c_uint64), ("peaks", C.c_uint64), ("fallbacks", C.c_uint64),
         ("ions_scanned", C.c_uint64), ("errors", C.c_uint64), ("items", C.c_uint64), ("chunks", C.c_uint32), ("config", C.c_int32),
         ("kernel_ms", C.c_double), ("evalue_ms", C.c_double), ("plan_ms", C.c_double), ("total_ms", C.c_double),
+    ]
+
+
+GX_TUNE_MAX = 16
+
+
+class gx_tune_result(C.Structure):
+    _fields_ = [
+        ("config", C.c_int32), ("split", C.c_int32), ("n_tried", C.c_int32),
+        ("tried_config", C.c_int32 * GX_TUNE_MAX), ("tried_split", C.c_int32 * GX_TUNE_MAX),
+        ("tried_ms", C.c_double * GX_TUNE_MAX),
     ]
 
 
@@ -197,5 +213,7 @@ def hip_lib() -> C.CDLL:
         lib.gx_search.argtypes = [vp, vp, C.POINTER(gx_query_batch), C.POINTER(gx_search_params), vp, vp,
                                   C.POINTER(gx_search_stats)]
         lib.gx_ctx_device_info.argtypes = [vp, C.c_char_p, C.c_size_t, C.POINTER(C.c_int)]
+        lib.gx_autotune.argtypes = [vp, vp, vp, C.POINTER(gx_search_params), C.POINTER(gx_tune_result)]
+        lib.gx_last_error.argtypes = [C.c_char_p, C.c_size_t]
         _HIP = lib
     return _HIP

@@ -30,7 +30,7 @@ struct PlanArgs {
     int dF;
     int nbins;
     double dM;
-    int tile_words;
+    int tile_words;          // count-tile words per item (the configuration's tile / split)
     // outputs of plan_spectra
     int32_t *lq;        // sorted legal peaks (CSR at qidx offsets)
     int32_t *li;
@@ -78,6 +78,7 @@ struct ItemArgs {
     uint32_t *work_counter;  // this chunk's counter
     uint32_t *cand_scratch;  // per block overflow candidates
     int tile_entries_max;
+    int split;               // window split: psids per item = tile / split
     // per-spectrum accumulators
     uint32_t *hist;          // n_spectra * GX_HIST_BINS
     uint32_t *cpsms;

@@ -362,7 +362,7 @@ __global__ void __launch_bounds__(THREADS) search_items_kernel(ItemArgs a) {
         const uint32_t lp = pack == 4 ? 2u : (pack == 2 ? 1u : 0u);  // log2 entries per word
         const uint32_t fbits = 32u >> lp;                             // counter width
         const uint32_t fmask = fbits == 32 ? 0xffffffffu : ((1u << fbits) - 1u);
-        const uint32_t T = (uint32_t)TILE_WORDS << lp;
+        const uint32_t T = ((uint32_t)TILE_WORDS / (uint32_t)a.split) << lp;
         const int32_t tlo = lo + (int32_t)(t * T);
         const int32_t thi = min(hi, tlo + (int32_t)T - 1);
         const uint32_t nent = (uint32_t)(thi - tlo + 1);

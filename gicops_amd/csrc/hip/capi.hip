@@ -23,7 +23,9 @@ struct gx_ctx {
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr, ev2 = nullptr, ev3 = nullptr;
     std::vector<hipEvent_t> ev_chunk;  // start/stop per planning chunk
-    int tuned_config = 0;
+    int tuned_config = 1;
+    int tuned_split = 1;
+    uint64_t scratch_gen = 0;  // bumped whenever d_slices / d_cand move (captured graphs hold them)
     double *d_lgfact = nullptr;
     int lgfact_n = 0;
     uint32_t *d_counter = nullptr;
@@ -80,14 +82,31 @@ struct gx_dqueries {
     uint32_t *pos = nullptr;
     size_t pos_cap = 0;  // words
     std::vector<uint64_t> h_pos_off;
+    // chunking of the last eager run, and the graph captured from it
+    std::vector<uint32_t> bounds, h_item_off;
+    uint32_t n_items_total = 0;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t graph_exec = nullptr;
+    std::vector<char> graph_key;
+    uint64_t buf_gen = 0;  // bumped whenever a chunk buffer moves
+    uint64_t graph_ctx_gen = 0, graph_buf_gen = 0;
 };
 
 namespace {
 
+thread_local char g_last_error[512] = "";
+
+void note_error(const char *what, int line, hipError_t e) {
+    std::snprintf(g_last_error, sizeof(g_last_error), "%s (capi.hip:%d): %s", what, line, hipGetErrorString(e));
+}
+
 #define GX_HIP(call)                                   \
     do {                                               \
         hipError_t e_ = (call);                        \
-        if (e_ != hipSuccess) return GX_ERR_DEVICE;    \
+        if (e_ != hipSuccess) {                        \
+            note_error(#call, __LINE__, e_);           \
+            return GX_ERR_DEVICE;                      \
+        }                                              \
     } while (0)
 
 template <class T>
@@ -104,6 +123,7 @@ int dalloc(T **p, size_t n) {
 int ensure_scratch(gx_ctx *c, int n_blocks, uint32_t max_slices) {
     const size_t sw = (size_t)n_blocks * 2 * max_slices;
     if (sw > c->slices_words) {
+        c->scratch_gen++;
         if (c->d_slices) (void)hipFree(c->d_slices);
         c->d_slices = nullptr;
         c->slices_words = 0;
@@ -113,6 +133,7 @@ int ensure_scratch(gx_ctx *c, int n_blocks, uint32_t max_slices) {
     }
     const size_t cw = (size_t)n_blocks * kTileEntries;
     if (cw > c->cand_words) {
+        c->scratch_gen++;
         if (c->d_cand) (void)hipFree(c->d_cand);
         c->d_cand = nullptr;
         c->cand_words = 0;
@@ -350,6 +371,8 @@ extern "C" void gx_queries_free(gx_dqueries *d) {
     if (!d) return;
     (void)hipSetDevice(d->ctx->device);
     (void)hipStreamSynchronize(d->ctx->stream);
+    if (d->graph_exec) (void)hipGraphExecDestroy(d->graph_exec);
+    if (d->graph) (void)hipGraphDestroy(d->graph);
     void *ps[] = {d->idx,       d->moz,        d->inten,         d->prec,         d->dm,       d->psm,
                   d->hist,      d->lq,         d->li,            d->n_legal,      d->pair_lo,  d->pair_hi,
                   d->pair_tiles, d->pair_pack, d->pair_item_off, d->pair_pos_off, d->n_items,  d->item_off,
@@ -462,8 +485,9 @@ int ensure_batched(gx_dqueries *q, const gx_dindex *idx) {
 }
 
 template <class T>
-int grow(T **p, size_t &cap, size_t n) {
+int grow(T **p, size_t &cap, size_t n, uint64_t &gen) {
     if (n <= cap && *p) return GX_SUCCESS;
+    gen++;
     if (*p) (void)hipFree(*p);
     *p = nullptr;
     cap = 0;
@@ -475,14 +499,23 @@ int grow(T **p, size_t &cap, size_t n) {
 
 constexpr uint64_t kPosChunkWords = 1ull << 27;  // 512 MB of slice positions per planning chunk
 
-int search_batched(gx_ctx *c, const gx_dindex *idx, gx_dqueries *q, const gx_search_params *p,
-                   gx_search_stats *stats) {
-    int st = ensure_batched(q, idx);
-    if (st) return st;
-    const int cfg_id = p->config >= 0 ? std::min(p->config, kNumBatchConfigs - 1) : c->tuned_config;
-    const BatchConfig cfg = batch_config(cfg_id);
-    const uint32_t ns = q->n_spectra;
-    PlanArgs pa{};
+struct Batched {
+    int cfg_id;
+    int split;
+    BatchConfig cfg;
+    PlanArgs pa;
+    ItemArgs ia;
+    EvalArgs ea;
+    int n_blocks;
+};
+
+Batched setup_batched(gx_ctx *c, const gx_dindex *idx, gx_dqueries *q, const gx_search_params *p) {
+    Batched B{};
+    B.cfg_id = p->config >= 0 ? std::min(p->config, kNumBatchConfigs - 1) : c->tuned_config;
+    B.split = p->split > 0 ? p->split : c->tuned_split;
+    if (B.split != 1 && B.split != 2 && B.split != 4) B.split = 1;
+    B.cfg = batch_config(B.cfg_id);
+    PlanArgs &pa = B.pa;
     pa.buckets = idx->d_buckets;
     pa.n_buckets = idx->n_buckets;
     pa.qidx = q->idx;
@@ -490,11 +523,11 @@ int search_batched(gx_ctx *c, const gx_dindex *idx, gx_dqueries *q, const gx_sea
     pa.inten = q->inten;
     pa.prec = q->prec;
     pa.dm = q->has_dm ? q->dm : nullptr;
-    pa.n_spectra = ns;
+    pa.n_spectra = q->n_spectra;
     pa.dF = p->dF;
     pa.nbins = (int)idx->nbins;
     pa.dM = p->dM;
-    pa.tile_words = cfg.tile_words;
+    pa.tile_words = B.cfg.tile_words / B.split;
     pa.lq = q->lq;
     pa.li = q->li;
     pa.n_legal = q->n_legal;
@@ -509,71 +542,12 @@ int search_batched(gx_ctx *c, const gx_dindex *idx, gx_dqueries *q, const gx_sea
     pa.item_off = q->item_off;
     pa.pos_off = q->pos_off;
     pa.stats = c->d_stats;
-    hipStream_t sm = c->stream;
-    GX_HIP(launch_fill_u32(reinterpret_cast<uint32_t *>(c->d_stats), (uint64_t)kStatRows * ST_N * 2, 0, sm));
-    GX_HIP(launch_fill_u32(q->hist, (uint64_t)ns * GX_HIST_BINS, 0, sm));
-    GX_HIP(launch_fill_u32(q->cpsms, ns, 0, sm));
-    GX_HIP(hipEventRecord(c->ev0, sm));
-    GX_HIP(launch_plan_spectra(pa, sm));
-    GX_HIP(launch_scan(pa, sm));
-    GX_HIP(hipEventRecord(c->ev1, sm));
-    // chunking needs the slice-position offsets on the host
-    uint32_t n_items_total = 0;
-    GX_HIP(hipMemcpyAsync(q->h_pos_off.data(), q->pos_off, (ns + 1) * sizeof(uint64_t), hipMemcpyDeviceToHost, sm));
-    GX_HIP(hipMemcpyAsync(&n_items_total, q->item_off + ns, sizeof(uint32_t), hipMemcpyDeviceToHost, sm));
-    GX_HIP(hipStreamSynchronize(sm));
-    std::vector<uint32_t> bounds{0};
-    uint64_t max_chunk = 1;
-    {
-        uint32_t s0 = 0;
-        while (s0 < ns) {
-            uint32_t s1 = s0 + 1;
-            while (s1 < ns && q->h_pos_off[s1 + 1] - q->h_pos_off[s0] <= kPosChunkWords) s1++;
-            max_chunk = std::max<uint64_t>(max_chunk, q->h_pos_off[s1] - q->h_pos_off[s0]);
-            bounds.push_back(s1);
-            s0 = s1;
-        }
-    }
-    const size_t n_chunks = bounds.size() - 1;
-    if (grow(&q->pos, q->pos_cap, max_chunk) || grow(&q->items, q->items_cap, 2 * (size_t)n_items_total + 2))
-        return GX_ERR_BAD_MEM_ALLOC;
-    if (q->items_cap / 2 + 1 > q->slots_cap || !q->item_key) {
-        // per-item slots follow the item capacity
-        if (q->item_key) (void)hipFree(q->item_key);
-        if (q->item_eq) (void)hipFree(q->item_eq);
-        if (q->item_shp) (void)hipFree(q->item_shp);
-        q->item_key = nullptr;
-        q->item_eq = q->item_shp = nullptr;
-        const size_t n = q->items_cap / 2 + 1;
-        if (dalloc(&q->item_key, n) || dalloc(&q->item_eq, n) || dalloc(&q->item_shp, n)) return GX_ERR_BAD_MEM_ALLOC;
-        q->slots_cap = n;
-    }
-    if (!q->counters || n_chunks > q->counters_n) {
-        if (q->counters) (void)hipFree(q->counters);
-        q->counters = nullptr;
-        q->counters_n = 0;
-        if (dalloc(&q->counters, std::max<size_t>(64, n_chunks))) return GX_ERR_BAD_MEM_ALLOC;
-        q->counters_n = std::max<size_t>(64, n_chunks);
-    }
-    const int n_blocks = std::max(1, c->n_cu * cfg.blocks_per_cu);
-    const size_t tile_entries_max = (size_t)cfg.tile_words * 4;
-    if (ensure_scratch(c, n_blocks, 1)) return GX_ERR_BAD_MEM_ALLOC;
-    if (c->cand_words < (size_t)n_blocks * tile_entries_max) {
-        if (c->d_cand) (void)hipFree(c->d_cand);
-        c->d_cand = nullptr;
-        c->cand_words = 0;
-        if (dalloc(&c->d_cand, (size_t)n_blocks * tile_entries_max)) return GX_ERR_BAD_MEM_ALLOC;
-        c->cand_words = (size_t)n_blocks * tile_entries_max;
-    }
-    while (c->ev_chunk.size() < 2 * n_chunks) {
-        hipEvent_t e;
-        GX_HIP(hipEventCreate(&e));
-        c->ev_chunk.push_back(e);
-    }
-    GX_HIP(launch_fill_u32(q->counters, std::max<size_t>(64, n_chunks), 0, sm));
-    pa.pos = q->pos;
-    pa.items = q->items;
-    ItemArgs ia{};
+    B.n_blocks = std::max(1, c->n_cu * B.cfg.blocks_per_cu);
+    return B;
+}
+
+void fill_item_args(Batched &B, gx_ctx *c, const gx_dindex *idx, gx_dqueries *q, const gx_search_params *p) {
+    ItemArgs &ia = B.ia;
     ia.buckets = idx->d_buckets;
     ia.n_buckets = idx->n_buckets;
     ia.qidx = q->idx;
@@ -593,33 +567,17 @@ int search_batched(gx_ctx *c, const gx_dindex *idx, gx_dqueries *q, const gx_sea
     ia.pos = q->pos;
     ia.items = q->items;
     ia.cand_scratch = c->d_cand;
-    ia.tile_entries_max = (int)tile_entries_max;
+    ia.tile_entries_max = B.cfg.tile_words * 4;
+    ia.split = B.split;
     ia.hist = q->hist;
     ia.cpsms = q->cpsms;
     ia.item_key = q->item_key;
     ia.item_eq = q->item_eq;
     ia.item_shp = q->item_shp;
     ia.stats = c->d_stats;
-    std::vector<uint32_t> h_item_off(n_chunks + 1);
-    for (size_t k = 0; k <= n_chunks; k++)
-        GX_HIP(hipMemcpyAsync(&h_item_off[k], q->item_off + bounds[k], sizeof(uint32_t), hipMemcpyDeviceToHost, sm));
-    GX_HIP(hipStreamSynchronize(sm));
-    for (size_t k = 0; k < n_chunks; k++) {
-        pa.chunk_s0 = bounds[k];
-        pa.chunk_s1 = bounds[k + 1];
-        GX_HIP(hipEventRecord(c->ev_chunk[2 * k], sm));
-        GX_HIP(launch_plan_slices(pa, sm));
-        GX_HIP(hipEventRecord(c->ev_chunk[2 * k + 1], sm));
-        ia.item0 = h_item_off[k];
-        ia.item1 = h_item_off[k + 1];
-        ia.chunk_s0 = bounds[k];
-        ia.work_counter = q->counters + k;
-        if (ia.item1 > ia.item0) GX_HIP(launch_search_items(cfg_id, ia, n_blocks, sm));
-    }
-    GX_HIP(hipEventRecord(c->ev2, sm));
-    EvalArgs ea{};
+    EvalArgs &ea = B.ea;
     ea.buckets = idx->d_buckets;
-    ea.n_spectra = ns;
+    ea.n_spectra = q->n_spectra;
     ea.min_cpsm = p->min_cpsm;
     ea.expect_max = p->expect_max;
     ea.item_off = q->item_off;
@@ -631,7 +589,154 @@ int search_batched(gx_ctx *c, const gx_dindex *idx, gx_dqueries *q, const gx_sea
     ea.route = nullptr;
     ea.out = q->psm;
     ea.stats = c->d_stats;
-    GX_HIP(launch_evalue(ea, sm));
+}
+
+// the device work before chunking: clear accumulators, plan spectra, scan
+hipError_t enqueue_prefix(gx_ctx *c, gx_dqueries *q, const Batched &B, hipStream_t sm) {
+    hipError_t e;
+    if ((e = launch_fill_u32(reinterpret_cast<uint32_t *>(c->d_stats), (uint64_t)kStatRows * ST_N * 2, 0, sm)))
+        return e;
+    if ((e = launch_fill_u32(q->hist, (uint64_t)q->n_spectra * GX_HIST_BINS, 0, sm))) return e;
+    if ((e = launch_fill_u32(q->cpsms, q->n_spectra, 0, sm))) return e;
+    if ((e = launch_plan_spectra(B.pa, sm))) return e;
+    return launch_scan(B.pa, sm);
+}
+
+// the per-chunk plan + search launches and the e-value kernel; per-chunk
+// events only on eager runs
+hipError_t enqueue_chunks(gx_ctx *c, gx_dqueries *q, Batched &B, hipStream_t sm, bool events) {
+    hipError_t e;
+    const size_t n_chunks = q->bounds.size() - 1;
+    if ((e = launch_fill_u32(q->counters, std::max<size_t>(64, n_chunks), 0, sm))) return e;
+    B.pa.pos = q->pos;
+    B.pa.items = q->items;
+    for (size_t k = 0; k < n_chunks; k++) {
+        B.pa.chunk_s0 = q->bounds[k];
+        B.pa.chunk_s1 = q->bounds[k + 1];
+        if (events && (e = hipEventRecord(c->ev_chunk[2 * k], sm))) return e;
+        if ((e = launch_plan_slices(B.pa, sm))) return e;
+        if (events && (e = hipEventRecord(c->ev_chunk[2 * k + 1], sm))) return e;
+        B.ia.item0 = q->h_item_off[k];
+        B.ia.item1 = q->h_item_off[k + 1];
+        B.ia.chunk_s0 = q->bounds[k];
+        B.ia.work_counter = q->counters + k;
+        if (B.ia.item1 > B.ia.item0 && (e = launch_search_items(B.cfg_id, B.ia, B.n_blocks, sm))) return e;
+    }
+    if (events && (e = hipEventRecord(c->ev2, sm))) return e;
+    return launch_evalue(B.ea, sm);
+}
+
+std::vector<char> graph_key_of(const Batched &B, const gx_search_params *p) {
+    std::vector<char> k(sizeof(gx_search_params) + 2 * sizeof(int));
+    gx_search_params pp = *p;
+    pp.config = B.cfg_id;
+    pp.split = B.split;
+    pp.use_graph = 0;
+    std::memcpy(k.data(), &pp, sizeof(pp));
+    return k;
+}
+
+int search_batched(gx_ctx *c, const gx_dindex *idx, gx_dqueries *q, const gx_search_params *p,
+                   gx_search_stats *stats) {
+    int st = ensure_batched(q, idx);
+    if (st) return st;
+    Batched B = setup_batched(c, idx, q, p);
+    const uint32_t ns = q->n_spectra;
+    hipStream_t sm = c->stream;
+    const std::vector<char> key = graph_key_of(B, p);
+
+    // ---- replay a captured graph of the same batch and parameters ----
+    if (p->use_graph && q->graph_exec && q->graph_key == key && q->graph_ctx_gen == c->scratch_gen &&
+        q->graph_buf_gen == q->buf_gen) {
+        GX_HIP(hipEventRecord(c->ev0, sm));
+        GX_HIP(hipGraphLaunch(q->graph_exec, sm));
+        GX_HIP(hipEventRecord(c->ev3, sm));
+        GX_HIP(hipStreamSynchronize(sm));
+        if (stats) {
+            unsigned long long h[ST_N];
+            read_stats(c, h);
+            fill_stats(stats, h);
+            stats->items = q->n_items_total;
+            stats->chunks = (uint32_t)(q->bounds.size() - 1);
+            stats->config = B.cfg_id;
+            stats->total_ms = elapsed(c->ev0, c->ev3);
+        }
+        return GX_SUCCESS;
+    }
+
+    // ---- eager run: plan, size the chunk buffers, search, model ----
+    GX_HIP(hipEventRecord(c->ev0, sm));
+    GX_HIP(enqueue_prefix(c, q, B, sm));
+    GX_HIP(hipEventRecord(c->ev1, sm));
+    // chunking needs the slice-position offsets on the host
+    uint32_t n_items_total = 0;
+    GX_HIP(hipMemcpyAsync(q->h_pos_off.data(), q->pos_off, (ns + 1) * sizeof(uint64_t), hipMemcpyDeviceToHost, sm));
+    GX_HIP(hipMemcpyAsync(&n_items_total, q->item_off + ns, sizeof(uint32_t), hipMemcpyDeviceToHost, sm));
+    GX_HIP(hipStreamSynchronize(sm));
+    q->bounds.assign(1, 0);
+    uint64_t max_chunk = 1;
+    {
+        uint32_t s0 = 0;
+        while (s0 < ns) {
+            uint32_t s1 = s0 + 1;
+            while (s1 < ns && q->h_pos_off[s1 + 1] - q->h_pos_off[s0] <= kPosChunkWords) s1++;
+            max_chunk = std::max<uint64_t>(max_chunk, q->h_pos_off[s1] - q->h_pos_off[s0]);
+            q->bounds.push_back(s1);
+            s0 = s1;
+        }
+    }
+    const size_t n_chunks = q->bounds.size() - 1;
+    if (grow(&q->pos, q->pos_cap, max_chunk, q->buf_gen) ||
+        grow(&q->items, q->items_cap, 2 * (size_t)n_items_total + 2, q->buf_gen))
+        return GX_ERR_BAD_MEM_ALLOC;
+    if (q->items_cap / 2 + 1 > q->slots_cap || !q->item_key) {
+        q->buf_gen++;
+        // per-item slots follow the item capacity
+        if (q->item_key) (void)hipFree(q->item_key);
+        if (q->item_eq) (void)hipFree(q->item_eq);
+        if (q->item_shp) (void)hipFree(q->item_shp);
+        q->item_key = nullptr;
+        q->item_eq = q->item_shp = nullptr;
+        const size_t n = q->items_cap / 2 + 1;
+        if (dalloc(&q->item_key, n) || dalloc(&q->item_eq, n) || dalloc(&q->item_shp, n)) return GX_ERR_BAD_MEM_ALLOC;
+        q->slots_cap = n;
+    }
+    if (!q->counters || n_chunks > q->counters_n) {
+        q->buf_gen++;
+        if (q->counters) (void)hipFree(q->counters);
+        q->counters = nullptr;
+        q->counters_n = 0;
+        if (dalloc(&q->counters, std::max<size_t>(64, n_chunks))) return GX_ERR_BAD_MEM_ALLOC;
+        q->counters_n = std::max<size_t>(64, n_chunks);
+    }
+    // candidate overflow scratch, sized once for every configuration so that
+    // it never moves under a captured graph
+    size_t cand_need = 0;
+    for (int k = 0; k < kNumBatchConfigs; k++) {
+        const BatchConfig bc = batch_config(k);
+        cand_need = std::max(cand_need, (size_t)std::max(1, c->n_cu * bc.blocks_per_cu) * bc.tile_words * 4);
+    }
+    if (c->cand_words < cand_need) {
+        c->scratch_gen++;
+        if (c->d_cand) (void)hipFree(c->d_cand);
+        c->d_cand = nullptr;
+        c->cand_words = 0;
+        if (dalloc(&c->d_cand, cand_need)) return GX_ERR_BAD_MEM_ALLOC;
+        c->cand_words = cand_need;
+    }
+    while (c->ev_chunk.size() < 2 * n_chunks) {
+        hipEvent_t e;
+        GX_HIP(hipEventCreate(&e));
+        c->ev_chunk.push_back(e);
+    }
+    q->h_item_off.assign(n_chunks + 1, 0);
+    for (size_t k = 0; k <= n_chunks; k++)
+        GX_HIP(hipMemcpyAsync(&q->h_item_off[k], q->item_off + q->bounds[k], sizeof(uint32_t), hipMemcpyDeviceToHost,
+                              sm));
+    GX_HIP(hipStreamSynchronize(sm));
+    q->n_items_total = n_items_total;
+    fill_item_args(B, c, idx, q, p);
+    GX_HIP(enqueue_chunks(c, q, B, sm, true));
     GX_HIP(hipEventRecord(c->ev3, sm));
     GX_HIP(hipStreamSynchronize(sm));
     q->searched = true;
@@ -641,7 +746,7 @@ int search_batched(gx_ctx *c, const gx_dindex *idx, gx_dqueries *q, const gx_sea
         fill_stats(stats, h);
         stats->items = n_items_total;
         stats->chunks = (uint32_t)n_chunks;
-        stats->config = cfg_id;
+        stats->config = B.cfg_id;
         double plan = elapsed(c->ev0, c->ev1), search = 0;
         for (size_t k = 0; k < n_chunks; k++) {
             plan += elapsed(c->ev_chunk[2 * k], c->ev_chunk[2 * k + 1]);
@@ -652,6 +757,38 @@ int search_batched(gx_ctx *c, const gx_dindex *idx, gx_dqueries *q, const gx_sea
         stats->kernel_ms = search;
         stats->evalue_ms = elapsed(c->ev2, c->ev3);
         stats->total_ms = elapsed(c->ev0, c->ev3);
+    }
+
+    // ---- capture the same launches (no host synchronisation) as a graph ----
+    if (p->use_graph) {
+        if (q->graph_exec) (void)hipGraphExecDestroy(q->graph_exec);
+        if (q->graph) (void)hipGraphDestroy(q->graph);
+        q->graph_exec = nullptr;
+        q->graph = nullptr;
+        q->graph_key.clear();
+        GX_HIP(hipStreamBeginCapture(sm, hipStreamCaptureModeThreadLocal));
+        hipError_t e = enqueue_prefix(c, q, B, sm);
+        if (e == hipSuccess) e = enqueue_chunks(c, q, B, sm, false);
+        hipGraph_t g = nullptr;
+        const hipError_t e2 = hipStreamEndCapture(sm, &g);
+        if (e != hipSuccess || e2 != hipSuccess || !g) {
+            note_error(e != hipSuccess ? "enqueue during capture" : "hipStreamEndCapture", __LINE__,
+                       e != hipSuccess ? e : e2);
+            if (g) (void)hipGraphDestroy(g);
+            return GX_ERR_DEVICE;
+        }
+        hipGraphExec_t ge = nullptr;
+        const hipError_t e3 = hipGraphInstantiate(&ge, g, nullptr, nullptr, 0);
+        if (e3 != hipSuccess) {
+            note_error("hipGraphInstantiate", __LINE__, e3);
+            (void)hipGraphDestroy(g);
+            return GX_ERR_DEVICE;
+        }
+        q->graph = g;
+        q->graph_exec = ge;
+        q->graph_key = key;
+        q->graph_ctx_gen = c->scratch_gen;
+        q->graph_buf_gen = q->buf_gen;
     }
     return GX_SUCCESS;
 }
@@ -704,4 +841,52 @@ extern "C" int gx_search(gx_ctx *c, const gx_dindex *idx, const gx_query_batch *
     if (!st) st = gx_results_download(c, q, psms, hist);
     gx_queries_free(q);
     return st;
+}
+
+extern "C" int gx_autotune(gx_ctx *c, const gx_dindex *idx, gx_dqueries *q, const gx_search_params *p,
+                           gx_tune_result *out) {
+    if (!c || !idx || !q || !p || !out) return GX_ERR_INVLD_PTR;
+    std::lock_guard<std::mutex> lk(c->mu);
+    std::memset(out, 0, sizeof(*out));
+    gx_search_params pp = *p;
+    pp.engine = GX_ENGINE_BATCHED;
+    pp.use_graph = 0;
+    double best = 1e300;
+    int bc = c->tuned_config, bs = c->tuned_split;
+    const int splits[] = {1, 2};
+    for (int cfg = 0; cfg < kNumBatchConfigs; cfg++) {
+        for (int sp : splits) {
+            pp.config = cfg;
+            pp.split = sp;
+            double t = 1e300;
+            for (int rep = 0; rep < 2; rep++) {  // the first run also sizes buffers
+                gx_search_stats st;
+                const int r = search_locked(c, idx, q, &pp, &st, false);
+                if (r) return r;
+                t = std::min(t, st.total_ms);
+            }
+            if (out->n_tried < GX_TUNE_MAX) {
+                out->tried_config[out->n_tried] = cfg;
+                out->tried_split[out->n_tried] = sp;
+                out->tried_ms[out->n_tried] = t;
+                out->n_tried++;
+            }
+            if (t < best) {
+                best = t;
+                bc = cfg;
+                bs = sp;
+            }
+        }
+    }
+    c->tuned_config = bc;
+    c->tuned_split = bs;
+    out->config = bc;
+    out->split = bs;
+    return GX_SUCCESS;
+}
+
+extern "C" int gx_last_error(char *buf, size_t len) {
+    if (!buf || !len) return GX_ERR_INVLD_PTR;
+    std::snprintf(buf, len, "%s", g_last_error);
+    return GX_SUCCESS;
 }

@@ -36,6 +36,7 @@ class SearchParams:
     engine: int = F.GX_ENGINE_BATCHED
     config: int = -1
     use_graph: bool = False
+    split: int = 0
 
     def to_ffi(self) -> F.gx_search_params:
         p = F.gx_search_params()
@@ -43,6 +44,7 @@ class SearchParams:
         p.min_shp, p.min_cpsm, p.evalue_mode = self.min_shp, self.min_cpsm, self.evalue_mode
         p.dM, p.expect_max, p.surrogate_conf = self.dM, self.expect_max, self.surrogate_conf
         p.engine, p.config, p.use_graph = self.engine, self.config, int(self.use_graph)
+        p.split = self.split
         return p
 
 
@@ -101,6 +103,15 @@ class Searcher:
         F.check("gx_search_device",
                 self._lib.gx_search_device(self._ctx, self._didx, dq._h, C.byref(p), C.byref(st)))
         return stats_dict(st)
+
+    def autotune(self, sample: DeviceQueries, params: SearchParams) -> dict:
+        """Time every launch configuration x window split on `sample` and make
+        the fastest this searcher's default (config=-1, split=0)."""
+        r = F.gx_tune_result()
+        p = params.to_ffi()
+        F.check("gx_autotune", self._lib.gx_autotune(self._ctx, self._didx, sample._h, C.byref(p), C.byref(r)))
+        return {"config": r.config, "split": r.split,
+                "tried": [(r.tried_config[i], r.tried_split[i], r.tried_ms[i]) for i in range(r.n_tried)]}
 
     def download(self, dq: DeviceQueries, want_hist: bool = False):
         psm = np.zeros(dq.n_spectra, dtype=F.PSM_DTYPE)

@@ -111,6 +111,8 @@ typedef struct gx_search_params {
     int32_t engine;        /* GX_ENGINE_* */
     int32_t config;        /* batched launch configuration, -1 = the context's tuned choice */
     int32_t use_graph;     /* batched: 1 = capture the batch as a hipGraph once, then replay */
+    int32_t split;         /* batched: window split, psids per item = tile / split (1, 2 or 4; 0 = tuned) */
+    int32_t reserved;
 } gx_search_params;
 
 /* ---- per-spectrum result (hCell + Results + e-value) ----------------- */
@@ -192,6 +194,30 @@ int gx_results_download(gx_ctx *ctx, gx_dqueries *q, gx_psm *psms, uint32_t *his
 /* One call: upload `q`, search, model, download (s3::search semantics). */
 int gx_search(gx_ctx *ctx, const gx_dindex *idx, const gx_query_batch *q, const gx_search_params *p,
               gx_psm *psms, uint32_t *hist, gx_search_stats *stats);
+
+/* Autotuner result: the launch configuration and window split chosen, and
+ * the device time of every candidate tried on the sample batch. */
+#define GX_TUNE_MAX 16
+typedef struct gx_tune_result {
+    int32_t config;
+    int32_t split;
+    int32_t n_tried;
+    int32_t tried_config[GX_TUNE_MAX];
+    int32_t tried_split[GX_TUNE_MAX];
+    double tried_ms[GX_TUNE_MAX];
+} gx_tune_result;
+
+/* Time every batched launch configuration x window split on a sample batch
+ * (device-resident, as uploaded by gx_queries_upload) and make the fastest
+ * the context's default (params.config = -1, params.split = 0). No reference
+ * counterpart: the reference's launch shape is fixed (kernel.cu:588-598). */
+int gx_autotune(gx_ctx *ctx, const gx_dindex *idx, gx_dqueries *sample, const gx_search_params *p,
+                gx_tune_result *out);
+
+/* Text of the last HIP failure reported as GX_ERR_DEVICE on this thread:
+ * the call, its source line and the HIP error string (the reference prints
+ * the CUDA error and exits, cuda/driver.hpp:41-48). */
+int gx_last_error(char *buf, size_t len);
 
 /* Name of the device the context runs on and its CU count (diagnostics). */
 int gx_ctx_device_info(gx_ctx *ctx, char *name, size_t name_len, int *n_cu);

@@ -84,7 +84,7 @@ def default_params(dM: float = 10.0, **kw) -> F.gx_search_params:
     p.dF, p.scale, p.max_mass, p.min_shp, p.min_cpsm = 2, 100, 5000, 4, 4
     p.evalue_mode = F.GX_EVALUE_TAILFIT
     p.dM, p.expect_max, p.surrogate_conf = dM, 20.0, 0.5
-    p.engine, p.config, p.use_graph = 0, -1, 0
+    p.engine, p.config, p.use_graph, p.split = 0, -1, 0, 0
     for k, v in kw.items():
         setattr(p, k, v)
     return p

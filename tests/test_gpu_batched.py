@@ -1,0 +1,72 @@
+"""Batched form: hipGraph capture/replay, window split, autotuner."""
+import sys
+
+import numpy as np
+import pytest
+
+from gicops_amd import _ffi as F
+from gicops_amd import slm
+from gicops_amd.search import SearchParams
+from oracle import pyoracle as O
+
+pytestmark = pytest.mark.gpu
+
+from test_gpu_parity import check_stats, compare  # noqa: E402
+
+
+def say(msg):
+    print(msg, flush=True)
+
+
+def _uneven(index, n=200, seed=41):
+    cfg = slm.SpecConfig(n_spectra=n, seed=seed, peak_mode=F.GX_PEAKS_TOTAL_POWER, peaks_min=20, peaks_max=2000,
+                         power_alpha=1.5, charge_min=2, charge_max=3, qalen=2048, dm_choices=(10.0, 100.0, 500.0))
+    return slm.generate_spectra(index, cfg)
+
+
+@pytest.mark.parametrize("split", [2, 4])
+def test_window_split_parity(open_index, searcher_open, split):
+    say(f"case: window split {split}, uneven batch")
+    batch = _uneven(open_index)
+    g, hg, st = searcher_open.search(batch, SearchParams(dM=500.0, config=1, split=split), want_hist=True)
+    c, hc, cnt = O.search(open_index, batch, O.default_params(dM=500.0), want_hist=True, want_counters=True)
+    check_stats(st, cnt, batch.n_spectra)
+    compare(g, c, hg, hc, f"split {split}")
+
+
+def test_graph_replays_match_eager(open_index, searcher_open):
+    say("case: hipGraph capture and replays")
+    batch = _uneven(open_index, n=150, seed=43)
+    dq = searcher_open.upload(batch)
+    eager = SearchParams(dM=500.0, config=1)
+    searcher_open.search_device(dq, eager)
+    ref_psm, ref_hist = searcher_open.download(dq, want_hist=True)
+    graph = SearchParams(dM=500.0, config=1, use_graph=True)
+    searcher_open.search_device(dq, graph)  # eager run + capture
+    other = _uneven(open_index, n=60, seed=44)
+    for rep in range(4):
+        st = searcher_open.search_device(dq, graph)  # replay
+        psm, hist = searcher_open.download(dq, want_hist=True)
+        assert psm.tobytes() == ref_psm.tobytes(), f"replay {rep}"
+        assert np.array_equal(hist, ref_hist), f"replay {rep}"
+        assert st["errors"] == 0 and st["ions_scanned"] == st["ions"]
+        # eager work on the same context between replays
+        searcher_open.search(other, SearchParams(dM=500.0, config=0))
+    c, hc, _ = O.search(open_index, batch, O.default_params(dM=500.0), want_hist=True)
+    compare(ref_psm, c, ref_hist, hc, "graph")
+    dq.free()
+
+
+def test_autotune_choice_keeps_parity(open_index, searcher_open):
+    say("case: autotuner")
+    sample = _uneven(open_index, n=120, seed=45)
+    dq = searcher_open.upload(sample)
+    r = searcher_open.autotune(dq, SearchParams(dM=500.0))
+    dq.free()
+    assert r["config"] in (0, 1, 2) and r["split"] in (1, 2)
+    assert len(r["tried"]) == 6 and all(ms > 0 for _, _, ms in r["tried"])
+    batch = _uneven(open_index, n=100, seed=46)
+    g, hg, st = searcher_open.search(batch, SearchParams(dM=500.0), want_hist=True)  # tuned defaults
+    assert st["config"] == r["config"]
+    c, hc, _ = O.search(open_index, batch, O.default_params(dM=500.0), want_hist=True)
+    compare(g, c, hg, hc, "autotuned")
