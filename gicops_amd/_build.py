@@ -34,6 +34,8 @@ HIP_SOURCES = [
     CSRC / "hip" / "search.hip",
     CSRC / "hip" / "batched.hip",
     CSRC / "hip" / "capi.hip",
+    CSRC / "hip" / "mlp.hip",
+    CSRC / "hip" / "mlp_capi.hip",
 ]
 HIP_HEADERS = sorted((CSRC / "hip").glob("*.h")) + sorted(INCLUDE.glob("*.h"))
 

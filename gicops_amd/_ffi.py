@@ -105,6 +105,25 @@ class gx_tune_result(C.Structure):
     ]
 
 
+GX_TRAIN_MAX_EPOCHS = 64
+MLP_FEAT, MLP_HIDDEN, MLP_OUT_PAD, MLP_OUT = 128, 256, 16, 3
+
+
+class gx_train_params(C.Structure):
+    _fields_ = [
+        ("batch", C.c_int32), ("epochs", C.c_int32), ("lr", C.c_float), ("lr_decay", C.c_float),
+        ("conf_weight", C.c_float), ("tol", C.c_float), ("seed", C.c_uint64),
+    ]
+
+
+class gx_train_report(C.Structure):
+    _fields_ = [
+        ("samples", C.c_uint64), ("epochs", C.c_int32), ("pad", C.c_int32),
+        ("mse_slope", C.c_float * GX_TRAIN_MAX_EPOCHS), ("mse_u", C.c_float * GX_TRAIN_MAX_EPOCHS),
+        ("bce", C.c_float * GX_TRAIN_MAX_EPOCHS), ("within_tol", C.c_float * GX_TRAIN_MAX_EPOCHS),
+    ]
+
+
 class gx_mod_spec(C.Structure):
     _fields_ = [("residues", C.c_char * 8), ("mass", C.c_float), ("max_per_pep", C.c_int32)]
 
@@ -162,6 +181,20 @@ def ptr(a: np.ndarray | None) -> C.c_void_p | None:
     return C.c_void_p(a.ctypes.data)
 
 
+def fptr(a: np.ndarray | None):
+    if a is None:
+        return None
+    assert a.dtype == np.float32 and a.flags["C_CONTIGUOUS"]
+    return a.ctypes.data_as(C.POINTER(C.c_float))
+
+
+def u8ptr(a: np.ndarray | None):
+    if a is None:
+        return None
+    assert a.dtype == np.uint8 and a.flags["C_CONTIGUOUS"]
+    return a.ctypes.data_as(C.POINTER(C.c_uint8))
+
+
 # ---- library loading --------------------------------------------------------
 _HOST = None
 _HIP = None
@@ -215,5 +248,19 @@ def hip_lib() -> C.CDLL:
         lib.gx_ctx_device_info.argtypes = [vp, C.c_char_p, C.c_size_t, C.POINTER(C.c_int)]
         lib.gx_autotune.argtypes = [vp, vp, vp, C.POINTER(gx_search_params), C.POINTER(gx_tune_result)]
         lib.gx_last_error.argtypes = [C.c_char_p, C.c_size_t]
+        fp, u8p = C.POINTER(C.c_float), C.POINTER(C.c_uint8)
+        lib.gx_mlp_param_count.argtypes = []
+        lib.gx_mlp_create.argtypes = [vp, C.POINTER(vp)]
+        lib.gx_mlp_destroy.argtypes = [vp]
+        lib.gx_mlp_destroy.restype = None
+        lib.gx_mlp_init.argtypes = [vp, C.c_uint64]
+        lib.gx_mlp_set_params.argtypes = [vp, fp, C.c_size_t, fp]
+        lib.gx_mlp_get_params.argtypes = [vp, fp, C.c_size_t, fp]
+        lib.gx_mlp_add_training.argtypes = [vp, vp, C.POINTER(C.c_uint64)]
+        lib.gx_mlp_train.argtypes = [vp, C.POINTER(gx_train_params), C.POINTER(gx_train_report)]
+        lib.gx_mlp_grad.argtypes = [vp, fp, fp, fp, C.c_uint32, fp, C.c_float, C.c_float, fp, fp, fp]
+        lib.gx_mlp_predict.argtypes = [vp, fp, C.c_uint32, fp]
+        lib.gx_ctx_set_surrogate.argtypes = [vp, vp]
+        lib.gx_features_download.argtypes = [vp, vp, fp, fp, fp, fp, u8p, u8p]
         _HIP = lib
     return _HIP

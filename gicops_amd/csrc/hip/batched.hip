@@ -36,12 +36,6 @@ __device__ __forceinline__ void stat_add(unsigned long long *stats, int which, u
     if (v) atomicAdd(&stats[(blockIdx.x % kStatSpread) * ST_N + which], v);
 }
 
-// exact floor(raw / d) for raw < 2^32 and 1 <= d < 2^12: (raw + 0.5)/d is at
-// least 0.5/d away from an integer, far more than the double rounding error
-__device__ __forceinline__ uint32_t div_entry(uint32_t raw, double inv_d) {
-    return (uint32_t)(((double)raw + 0.5) * inv_d);
-}
-
 __global__ void fill_u32_kernel(uint32_t *p, uint64_t n, uint32_t v) {
     for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
         p[i] = v;
@@ -628,60 +622,68 @@ __global__ void __launch_bounds__(64) evalue_kernel(EvalArgs a) {
     __shared__ FitScratch F;
     const int lane = threadIdx.x;
     const uint32_t s = blockIdx.x;
-    const uint32_t i0 = a.item_off[s], i1 = a.item_off[s + 1];
-    unsigned long long key = 0;
-    for (uint32_t i = i0 + lane; i < i1; i += 64) {
-        const unsigned long long k = a.item_key[i];
-        key = k > key ? k : key;
+    const uint32_t need = (uint32_t)max(a.min_cpsm, 1);
+    gx_psm r;
+    if (a.mode == 2) {
+        // the surrogate's fallback: fit the modeled spectra it did not accept
+        r = a.out[s];
+        if (a.route[s] || !(r.flags & GX_PSM_MODELED)) return;
+    } else {
+        const uint32_t i0 = a.item_off[s], i1 = a.item_off[s + 1];
+        unsigned long long key = 0;
+        for (uint32_t i = i0 + lane; i < i1; i += 64) {
+            const unsigned long long k = a.item_key[i];
+            key = k > key ? k : key;
+        }
+        key = wave_max_u64(key);
+        uint32_t eq = 0, shp = 0;
+        for (uint32_t i = i0 + lane; i < i1; i += 64) {
+            const unsigned long long k = a.item_key[i];
+            if (k && (k >> 32) == (key >> 32)) eq += a.item_eq[i];
+            if (k && k == key) shp = a.item_shp[i];
+        }
+        eq = wave_sum_u32(eq);
+        shp = (uint32_t)wave_max_i32((int)shp);
+        const uint32_t cps = a.cpsms[s];
+        r.hyperscore = 0.0f;
+        r.psid = -1;
+        r.idxoffset = 0;
+        r.sharedions = 0;
+        r.totalions = 0;
+        r.flags = 0;
+        r.cpsms = (int32_t)cps;
+        r.hyp = 0;
+        r.mu = 0;
+        r.beta = 0;
+        r.evalue = INFINITY;
+        if (cps >= 1 && key) {
+            const uint32_t low = ~(uint32_t)(key & 0xffffffffu);
+            const uint32_t b = low >> 26, psid = low & ((1u << 26) - 1);
+            r.flags |= GX_PSM_HAS_HIT;
+            r.hyperscore = __uint_as_float((uint32_t)(key >> 32));
+            r.psid = (int32_t)psid;
+            r.idxoffset = (uint16_t)b;
+            r.sharedions = (uint16_t)shp;
+            r.totalions = (uint16_t)a.buckets[b].speclen;
+            r.hyp = hyp_bin(r.hyperscore);
+            if (eq > 1) r.flags |= GX_PSM_TIE;
+        }
+        if (cps >= need && r.psid >= 0) r.flags |= GX_PSM_MODELED;
+        if (a.mode == 1 || !(r.flags & GX_PSM_MODELED)) {
+            if (lane == 0) a.out[s] = r;
+            return;
+        }
     }
-    key = wave_max_u64(key);
-    uint32_t eq = 0, shp = 0;
-    for (uint32_t i = i0 + lane; i < i1; i += 64) {
-        const unsigned long long k = a.item_key[i];
-        if (k && (k >> 32) == (key >> 32)) eq += a.item_eq[i];
-        if (k && k == key) shp = a.item_shp[i];
-    }
-    eq = wave_sum_u32(eq);
-    shp = (uint32_t)wave_max_i32((int)shp);
     for (int w = lane; w < GX_HIST_BINS; w += 64) hist[w] = a.hist[(size_t)s * GX_HIST_BINS + w];
     __syncthreads();
-    const uint32_t cps = a.cpsms[s];
-    gx_psm r;
-    r.hyperscore = 0.0f;
-    r.psid = -1;
-    r.idxoffset = 0;
-    r.sharedions = 0;
-    r.totalions = 0;
-    r.flags = 0;
-    r.cpsms = (int32_t)cps;
-    r.hyp = 0;
-    r.mu = 0;
-    r.beta = 0;
-    r.evalue = INFINITY;
-    if (cps >= 1 && key) {
-        const uint32_t low = ~(uint32_t)(key & 0xffffffffu);
-        const uint32_t b = low >> 26, psid = low & ((1u << 26) - 1);
-        r.flags |= GX_PSM_HAS_HIT;
-        r.hyperscore = __uint_as_float((uint32_t)(key >> 32));
-        r.psid = (int32_t)psid;
-        r.idxoffset = (uint16_t)b;
-        r.sharedions = (uint16_t)shp;
-        r.totalions = (uint16_t)a.buckets[b].speclen;
-        r.hyp = hyp_bin(r.hyperscore);
-        if (eq > 1) r.flags |= GX_PSM_TIE;
-    }
-    const uint32_t need = (uint32_t)max(a.min_cpsm, 1);
-    if (cps >= need && r.psid >= 0) {
-        r.flags |= GX_PSM_MODELED;
-        double sl, ic;
-        int32_t mu, beta;
-        if (tail_fit_block(hist, sx, F, r.hyp, (int)cps, sl, ic, mu, beta)) {
-            r.flags |= GX_PSM_FIT_OK;
-            r.mu = mu;
-            r.beta = beta;
-            r.evalue = evalue_of(mu, beta, r.hyp, cps);
-            if (r.evalue < a.expect_max) r.flags |= GX_PSM_REPORTED;
-        }
+    double sl, ic;
+    int32_t mu, beta;
+    if (tail_fit_block(hist, sx, F, r.hyp, r.cpsms, sl, ic, mu, beta)) {
+        r.flags |= GX_PSM_FIT_OK;
+        r.mu = mu;
+        r.beta = beta;
+        r.evalue = evalue_of(mu, beta, r.hyp, (uint32_t)r.cpsms);
+        if (r.evalue < a.expect_max) r.flags |= GX_PSM_REPORTED;
     }
     if (lane == 0) a.out[s] = r;
 }

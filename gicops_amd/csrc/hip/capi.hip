@@ -10,115 +10,18 @@
 #include <new>
 #include <vector>
 
-#include "batched.h"
-#include "gicops_amd.h"
-#include "search_common.h"
+#include "capi_internal.h"
 
 using namespace gx;
 
-struct gx_ctx {
-    int device = 0;
-    int n_cu = 0;
-    char name[256] = {};
-    hipStream_t stream = nullptr;
-    hipEvent_t ev0 = nullptr, ev1 = nullptr, ev2 = nullptr, ev3 = nullptr;
-    std::vector<hipEvent_t> ev_chunk;  // start/stop per planning chunk
-    int tuned_config = 1;
-    int tuned_split = 1;
-    uint64_t scratch_gen = 0;  // bumped whenever d_slices / d_cand move (captured graphs hold them)
-    double *d_lgfact = nullptr;
-    int lgfact_n = 0;
-    uint32_t *d_counter = nullptr;
-    unsigned long long *d_stats = nullptr;
-    // per-block scratch, grown on demand
-    uint32_t *d_slices = nullptr;
-    size_t slices_words = 0;
-    uint32_t *d_cand = nullptr;
-    size_t cand_words = 0;
-    std::mutex mu;
-};
-
-struct gx_dindex {
-    gx_ctx *ctx = nullptr;
-    int n_buckets = 0;
-    int maxz = 1;
-    uint32_t nbins = 0;
-    uint32_t max_speclen = 0;
-    std::vector<void *> allocs;
-    DevBucket *d_buckets = nullptr;
-    std::vector<DevBucket> h_buckets;
-};
-
-struct gx_dqueries {
-    gx_ctx *ctx = nullptr;
-    uint32_t n_spectra = 0;
-    uint64_t n_peaks = 0;
-    uint32_t max_peaks = 0;
-    bool has_dm = false;
-    uint32_t *idx = nullptr;
-    int32_t *moz = nullptr;
-    int32_t *inten = nullptr;
-    float *prec = nullptr;
-    float *dm = nullptr;
-    gx_psm *psm = nullptr;
-    uint32_t *hist = nullptr;  // per-spectrum survival histograms (batched: always; plain: on request)
-    bool searched = false;
-    // batched engine workspace, sized on first use
-    int nb = 0;
-    int32_t *lq = nullptr, *li = nullptr;
-    uint32_t *n_legal = nullptr;
-    int32_t *pair_lo = nullptr, *pair_hi = nullptr;
-    uint32_t *pair_tiles = nullptr, *pair_pack = nullptr, *pair_item_off = nullptr, *pair_pos_off = nullptr;
-    uint32_t *n_items = nullptr, *item_off = nullptr;
-    uint64_t *n_pos = nullptr, *pos_off = nullptr;
-    uint32_t *cpsms = nullptr;
-    uint32_t *counters = nullptr;  // one work counter per chunk
-    size_t counters_n = 0;
-    size_t slots_cap = 0;          // per-item top-hit slots
-    uint32_t *items = nullptr;
-    size_t items_cap = 0;  // items
-    unsigned long long *item_key = nullptr;
-    uint32_t *item_eq = nullptr, *item_shp = nullptr;
-    uint32_t *pos = nullptr;
-    size_t pos_cap = 0;  // words
-    std::vector<uint64_t> h_pos_off;
-    // chunking of the last eager run, and the graph captured from it
-    std::vector<uint32_t> bounds, h_item_off;
-    uint32_t n_items_total = 0;
-    hipGraph_t graph = nullptr;
-    hipGraphExec_t graph_exec = nullptr;
-    std::vector<char> graph_key;
-    uint64_t buf_gen = 0;  // bumped whenever a chunk buffer moves
-    uint64_t graph_ctx_gen = 0, graph_buf_gen = 0;
-};
+namespace gx {
+thread_local char g_last_error[512] = "";
+void note_error(const char *what, int line, hipError_t e) {
+    std::snprintf(g_last_error, sizeof(g_last_error), "%s (line %d): %s", what, line, hipGetErrorString(e));
+}
+}  // namespace gx
 
 namespace {
-
-thread_local char g_last_error[512] = "";
-
-void note_error(const char *what, int line, hipError_t e) {
-    std::snprintf(g_last_error, sizeof(g_last_error), "%s (capi.hip:%d): %s", what, line, hipGetErrorString(e));
-}
-
-#define GX_HIP(call)                                   \
-    do {                                               \
-        hipError_t e_ = (call);                        \
-        if (e_ != hipSuccess) {                        \
-            note_error(#call, __LINE__, e_);           \
-            return GX_ERR_DEVICE;                      \
-        }                                              \
-    } while (0)
-
-template <class T>
-int dalloc(T **p, size_t n) {
-    *p = nullptr;
-    if (n == 0) n = 1;
-    if (hipMalloc((void **)p, n * sizeof(T)) != hipSuccess) {
-        *p = nullptr;
-        return GX_ERR_BAD_MEM_ALLOC;
-    }
-    return GX_SUCCESS;
-}
 
 int ensure_scratch(gx_ctx *c, int n_blocks, uint32_t max_slices) {
     const size_t sw = (size_t)n_blocks * 2 * max_slices;
@@ -377,7 +280,8 @@ extern "C" void gx_queries_free(gx_dqueries *d) {
                   d->hist,      d->lq,         d->li,            d->n_legal,      d->pair_lo,  d->pair_hi,
                   d->pair_tiles, d->pair_pack, d->pair_item_off, d->pair_pos_off, d->n_items,  d->item_off,
                   d->n_pos,     d->pos_off,    d->cpsms,         d->counters,     d->items,    d->item_key,
-                  d->item_eq,   d->item_shp,   d->pos};
+                  d->item_eq,   d->item_shp,   d->pos,           d->l_feat,       d->l_h1,     d->l_h2,
+                  d->l_anchor,  d->l_tgt,      d->l_loge,     d->l_out,         d->l_valid,      d->l_sparse, d->l_route};
     for (void *p : ps)
         if (p) (void)hipFree(p);
     delete d;
@@ -507,6 +411,8 @@ struct Batched {
     ItemArgs ia;
     EvalArgs ea;
     int n_blocks;
+    bool learned;  // e-values from the surrogate, plain fit as fallback
+    float conf;
 };
 
 Batched setup_batched(gx_ctx *c, const gx_dindex *idx, gx_dqueries *q, const gx_search_params *p) {
@@ -543,6 +449,8 @@ Batched setup_batched(gx_ctx *c, const gx_dindex *idx, gx_dqueries *q, const gx_
     pa.pos_off = q->pos_off;
     pa.stats = c->d_stats;
     B.n_blocks = std::max(1, c->n_cu * B.cfg.blocks_per_cu);
+    B.learned = p->evalue_mode == GX_EVALUE_LEARNED;
+    B.conf = p->surrogate_conf;
     return B;
 }
 
@@ -587,6 +495,7 @@ void fill_item_args(Batched &B, gx_ctx *c, const gx_dindex *idx, gx_dqueries *q,
     ea.hist = q->hist;
     ea.cpsms = q->cpsms;
     ea.route = nullptr;
+    ea.mode = 0;
     ea.out = q->psm;
     ea.stats = c->d_stats;
 }
@@ -623,6 +532,7 @@ hipError_t enqueue_chunks(gx_ctx *c, gx_dqueries *q, Batched &B, hipStream_t sm,
         if (B.ia.item1 > B.ia.item0 && (e = launch_search_items(B.cfg_id, B.ia, B.n_blocks, sm))) return e;
     }
     if (events && (e = hipEventRecord(c->ev2, sm))) return e;
+    if (B.learned) return enqueue_learned_evalue(c, q, B.ea, B.conf, sm);
     return launch_evalue(B.ea, sm);
 }
 
@@ -640,6 +550,7 @@ int search_batched(gx_ctx *c, const gx_dindex *idx, gx_dqueries *q, const gx_sea
                    gx_search_stats *stats) {
     int st = ensure_batched(q, idx);
     if (st) return st;
+    if (p->evalue_mode == GX_EVALUE_LEARNED && (st = ensure_learned(c, q))) return st;
     Batched B = setup_batched(c, idx, q, p);
     const uint32_t ns = q->n_spectra;
     hipStream_t sm = c->stream;
@@ -802,6 +713,12 @@ int search_locked(gx_ctx *c, const gx_dindex *idx, gx_dqueries *q, const gx_sear
     if (hipSetDevice(c->device) != hipSuccess) return GX_ERR_DEVICE;
     // every count bc or yc stays below (2dF+1)*speclen/2 + 1
     if ((int)((2 * p->dF + 1) * (idx->max_speclen / 2) + 1) > c->lgfact_n) return GX_ERR_INVLD_SIZE;
+    if (p->evalue_mode == GX_EVALUE_LEARNED) {
+        // the surrogate reads the batched engine's per-spectrum histograms
+        if (p->engine != GX_ENGINE_BATCHED) return GX_ERR_INVLD_PARAM;
+        if (!c->surrogate) return GX_ERR_NOT_READY;
+        if (!(p->surrogate_conf >= 0.0f && p->surrogate_conf <= 1.0f)) return GX_ERR_INVLD_PARAM;
+    }
     if (p->engine == GX_ENGINE_SPECTRUM) return search_plain(c, idx, q, p, stats, want_hist);
     return search_batched(c, idx, q, p, stats);
 }

@@ -1,0 +1,133 @@
+// Internal state shared by the C ABI translation units (not installed).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+#include <vector>
+
+#include "batched.h"
+#include "gicops_amd.h"
+#include "mlp.h"
+#include "search_common.h"
+
+namespace gx {
+struct Surrogate;
+}
+
+using namespace gx;
+
+struct gx_ctx {
+    int device = 0;
+    int n_cu = 0;
+    char name[256] = {};
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr, ev2 = nullptr, ev3 = nullptr;
+    std::vector<hipEvent_t> ev_chunk;  // start/stop per planning chunk
+    int tuned_config = 1;
+    int tuned_split = 1;
+    uint64_t scratch_gen = 0;  // bumped whenever d_slices / d_cand move (captured graphs hold them)
+    double *d_lgfact = nullptr;
+    int lgfact_n = 0;
+    uint32_t *d_counter = nullptr;
+    unsigned long long *d_stats = nullptr;
+    // per-block scratch, grown on demand
+    uint32_t *d_slices = nullptr;
+    size_t slices_words = 0;
+    uint32_t *d_cand = nullptr;
+    size_t cand_words = 0;
+    gx::Surrogate *surrogate = nullptr;  // learned e-value model (gx_ctx_set_surrogate)
+    std::mutex mu;
+};
+
+struct gx_dindex {
+    gx_ctx *ctx = nullptr;
+    int n_buckets = 0;
+    int maxz = 1;
+    uint32_t nbins = 0;
+    uint32_t max_speclen = 0;
+    std::vector<void *> allocs;
+    DevBucket *d_buckets = nullptr;
+    std::vector<DevBucket> h_buckets;
+};
+
+struct gx_dqueries {
+    gx_ctx *ctx = nullptr;
+    uint32_t n_spectra = 0;
+    uint64_t n_peaks = 0;
+    uint32_t max_peaks = 0;
+    bool has_dm = false;
+    uint32_t *idx = nullptr;
+    int32_t *moz = nullptr;
+    int32_t *inten = nullptr;
+    float *prec = nullptr;
+    float *dm = nullptr;
+    gx_psm *psm = nullptr;
+    uint32_t *hist = nullptr;  // per-spectrum survival histograms (batched: always; plain: on request)
+    bool searched = false;
+    // batched engine workspace, sized on first use
+    int nb = 0;
+    int32_t *lq = nullptr, *li = nullptr;
+    uint32_t *n_legal = nullptr;
+    int32_t *pair_lo = nullptr, *pair_hi = nullptr;
+    uint32_t *pair_tiles = nullptr, *pair_pack = nullptr, *pair_item_off = nullptr, *pair_pos_off = nullptr;
+    uint32_t *n_items = nullptr, *item_off = nullptr;
+    uint64_t *n_pos = nullptr, *pos_off = nullptr;
+    uint32_t *cpsms = nullptr;
+    uint32_t *counters = nullptr;  // one work counter per chunk
+    size_t counters_n = 0;
+    size_t slots_cap = 0;          // per-item top-hit slots
+    uint32_t *items = nullptr;
+    size_t items_cap = 0;  // items
+    unsigned long long *item_key = nullptr;
+    uint32_t *item_eq = nullptr, *item_shp = nullptr;
+    uint32_t *pos = nullptr;
+    size_t pos_cap = 0;  // words
+    std::vector<uint64_t> h_pos_off;
+    // chunking of the last eager run, and the graph captured from it
+    std::vector<uint32_t> bounds, h_item_off;
+    uint32_t n_items_total = 0;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t graph_exec = nullptr;
+    std::vector<char> graph_key;
+    uint64_t buf_gen = 0;  // bumped whenever a chunk buffer moves
+    uint64_t graph_ctx_gen = 0, graph_buf_gen = 0;
+    // learned e-value stage: features, targets, activations, routing
+    bf16 *l_feat = nullptr, *l_h1 = nullptr, *l_h2 = nullptr;
+    float *l_anchor = nullptr, *l_tgt = nullptr, *l_loge = nullptr, *l_out = nullptr;
+    uint8_t *l_valid = nullptr, *l_sparse = nullptr, *l_route = nullptr;
+};
+
+namespace gx {
+// text of the last HIP failure on this thread (gx_last_error)
+extern thread_local char g_last_error[512];
+void note_error(const char *what, int line, hipError_t e);
+}  // namespace gx
+
+#define GX_HIP(call)                                   \
+    do {                                               \
+        hipError_t e_ = (call);                        \
+        if (e_ != hipSuccess) {                        \
+            note_error(#call, __LINE__, e_);           \
+            return GX_ERR_DEVICE;                      \
+        }                                              \
+    } while (0)
+
+namespace gx {
+// learned e-value stage of a batched search (mlp_capi.hip)
+int ensure_learned(gx_ctx *c, gx_dqueries *q);
+hipError_t enqueue_learned_evalue(gx_ctx *c, gx_dqueries *q, EvalArgs ea, float conf, hipStream_t s);
+
+template <class T>
+int dalloc(T **p, size_t n) {
+    *p = nullptr;
+    if (n == 0) n = 1;
+    if (hipMalloc((void **)p, n * sizeof(T)) != hipSuccess) {
+        *p = nullptr;
+        return GX_ERR_BAD_MEM_ALLOC;
+    }
+    return GX_SUCCESS;
+}
+}  // namespace gx

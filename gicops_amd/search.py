@@ -87,6 +87,7 @@ class Searcher:
         F.check("gx_index_upload", self._lib.gx_index_upload(ctx, index.raw_array, len(index.buckets),
                                                               index.maxz, C.byref(d)))
         self._didx = d
+        self._children: list = []  # surrogates, closed before the context
 
     def device_info(self) -> tuple[str, int]:
         buf = C.create_string_buffer(256)
@@ -131,7 +132,28 @@ class Searcher:
                                                  F.ptr(hist), C.byref(st)))
         return psm, hist, stats_dict(st)
 
+    def features(self, dq: DeviceQueries) -> dict:
+        """Surrogate features and plain-fit targets of a searched batch
+        (gx_features_download): feat (n,128), anchor (n,), target (n,2) =
+        slope, u - anchor; loge (n,) = log10 of the plain e-value; valid,
+        sparse (n,) bool."""
+        n = dq.n_spectra
+        feat = np.zeros((n, F.MLP_FEAT), np.float32)
+        anchor = np.zeros(n, np.float32)
+        target = np.zeros((n, 2), np.float32)
+        loge = np.zeros(n, np.float32)
+        valid = np.zeros(n, np.uint8)
+        sparse = np.zeros(n, np.uint8)
+        F.check("gx_features_download", self._lib.gx_features_download(
+            self._ctx, dq._h, F.fptr(feat), F.fptr(anchor), F.fptr(target), F.fptr(loge), F.u8ptr(valid),
+            F.u8ptr(sparse)))
+        return {"feat": feat, "anchor": anchor, "target": target, "loge": loge, "valid": valid.astype(bool),
+                "sparse": sparse.astype(bool)}
+
     def close(self) -> None:
+        for ch in self._children:
+            ch.close()
+        self._children = []
         if getattr(self, "_didx", None):
             self._lib.gx_index_free(self._didx)
             self._didx = None

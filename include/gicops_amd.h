@@ -222,6 +222,91 @@ int gx_last_error(char *buf, size_t len);
 /* Name of the device the context runs on and its CU count (diagnostics). */
 int gx_ctx_device_info(gx_ctx *ctx, char *name, size_t name_len, int *n_cu);
 
+/* ---- learned e-value surrogate ----------------------------------------
+ * A 128-256-256-3 GELU MLP (BF16 operands on MFMA, FP32 accumulation and
+ * master weights) that maps a spectrum's survival histogram to the tail-fit
+ * model of expeRT::ModelTailFit (expeRT.cpp:396-559): slope, fitted
+ * log10-survival at the top hit's bin, and a confidence logit. With
+ * evalue_mode = GX_EVALUE_LEARNED the batched engine takes the surrogate's
+ * e-value where the confidence is at least surrogate_conf and the histogram
+ * has at least 3 non-empty bins below the top hit; every other modeled
+ * spectrum gets the plain tail fit (GX_PSM_SURROGATE tells them apart, and
+ * stats.fallbacks counts the plain fits). Top hits, cpsms and histograms are
+ * the same in both modes. No reference counterpart: the reference always
+ * runs the plain fit.
+ *
+ * Features (per spectrum, bf16): the plain fit's log10 survival curve
+ * (bins stt1..end1, from the first non-empty bin to the last non-empty one
+ * below the top hit, expeRT.cpp:414-475) resampled to 120 points by linear
+ * interpolation and divided by 4, then stt1/800, end1/800, hyp/800,
+ * (hyp-end1)/400, n/400, log10(cpsms)/4, anchor/4 and the first curve value,
+ * where the anchor is the curve's last value. Targets: slope = mu/1e6 and
+ * u - anchor with u = slope*hyp + beta/1e6 (the reference's truncated
+ * integers); the surrogate's e-value is cpsms * 10^(anchor + predicted
+ * residual). */
+typedef struct gx_mlp gx_mlp;
+
+#define GX_TRAIN_MAX_EPOCHS 64
+typedef struct gx_train_params {
+    int32_t batch;       /* minibatch rows (>= 16) */
+    int32_t epochs;      /* passes over the training set (1..GX_TRAIN_MAX_EPOCHS) */
+    float lr;            /* Adam step size (b1 0.9, b2 0.999, eps 1e-8) */
+    float lr_decay;      /* per-epoch multiplier of lr (0 or 1 = constant) */
+    float conf_weight;   /* weight of the confidence cross-entropy */
+    float tol;           /* confidence label: |u_pred - u| <= tol * max(|log10 e|, 1e-3) */
+    uint64_t seed;       /* minibatch shuffling */
+} gx_train_params;
+
+typedef struct gx_train_report {
+    uint64_t samples;    /* training rows */
+    int32_t epochs;      /* epochs run */
+    int32_t pad;
+    /* per epoch, means over the rows seen: squared errors of the normalised
+     * slope and u, confidence cross-entropy, fraction within tolerance */
+    float mse_slope[GX_TRAIN_MAX_EPOCHS];
+    float mse_u[GX_TRAIN_MAX_EPOCHS];
+    float bce[GX_TRAIN_MAX_EPOCHS];
+    float within_tol[GX_TRAIN_MAX_EPOCHS];
+} gx_train_report;
+
+/* Number of fp32 parameters: W1[128][256], b1[256], W2[256][256], b2[256],
+ * W3[256][16], b3[16] (row-major, in that order; output columns 3..15 are
+ * padding and stay zero). */
+int gx_mlp_param_count(void);
+int gx_mlp_create(gx_ctx *ctx, gx_mlp **out);
+void gx_mlp_destroy(gx_mlp *m);
+/* Glorot-uniform weights, zero biases, Adam state cleared. */
+int gx_mlp_init(gx_mlp *m, uint64_t seed);
+/* Set / read the parameters; `norm` (optional, 4 floats) is the output
+ * normalisation {slope shift, slope scale, u shift, u scale}: target =
+ * output * scale + shift. Setting parameters clears the Adam state. */
+int gx_mlp_set_params(gx_mlp *m, const float *params, size_t n, const float *norm);
+int gx_mlp_get_params(gx_mlp *m, float *params, size_t n, float *norm);
+/* Append the spectra of a batch searched in GX_EVALUE_TAILFIT mode whose
+ * plain fit is defined to the training set (features and targets stay in
+ * HBM). */
+int gx_mlp_add_training(gx_mlp *m, gx_dqueries *q, uint64_t *n_added);
+/* Train on the GPU (forward, loss, backward and Adam kernels). The output
+ * normalisation is fixed from the training targets on the first call unless
+ * set by gx_mlp_set_params. */
+int gx_mlp_train(gx_mlp *m, const gx_train_params *tp, gx_train_report *rep);
+/* One forward + backward pass on host rows (feat n*128, target n*2, loge n)
+ * with the given normalisation; writes the gradient of the mean loss
+ * (gx_mlp_param_count floats), and optionally the raw outputs (n*16) and the
+ * loss sums {sq. err slope, sq. err u, cross-entropy, rows within tol}. */
+int gx_mlp_grad(gx_mlp *m, const float *feat, const float *target, const float *loge, uint32_t n,
+                const float *norm, float conf_weight, float tol, float *grad, float *out, float *loss);
+/* Forward pass on host rows: out = n*16 raw outputs (columns 0..2 used). */
+int gx_mlp_predict(gx_mlp *m, const float *feat, uint32_t n, float *out);
+/* Attach the surrogate used by GX_EVALUE_LEARNED searches (NULL detaches). */
+int gx_ctx_set_surrogate(gx_ctx *ctx, gx_mlp *m);
+/* Features (n*128), anchors (n), targets (n*2), log10 of the plain e-value
+ * (n), and the valid / sparse flags of a searched batch (any output pointer
+ * may be NULL). Targets and log10 e are meaningful for spectra searched with
+ * GX_EVALUE_TAILFIT whose valid flag is set. */
+int gx_features_download(gx_ctx *ctx, gx_dqueries *q, float *feat, float *anchor, float *target, float *loge,
+                         uint8_t *valid, uint8_t *sparse);
+
 #ifdef __cplusplus
 }
 #endif
