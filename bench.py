@@ -5,13 +5,20 @@ Workload (default `open2m`): 2M synthetic tryptic peptides plus variable Met
 oxidation (<=2 per peptide, ~5.6M index entries, ~290M fragment ions, charge-1
 b/y, 0.01 Da bins), 100k query spectra (50-150 raw peaks, top 100 kept),
 dF = 0.02 Da, open search dM = +-500 Da. A step is one search of the whole
-batch (window, scan, scoring, histogram, top hit and tail-fit e-value for
-every spectrum) with the batch already resident in HBM.
+batch (window, scan, scoring, histogram, top hit and e-value for every
+spectrum) with the batch already resident in HBM.
 
-Prints one JSON line. `roofline.achieved` is the algorithmic bytes of a launch
-(8 B per query peak, 8 B per bA pair read, 4 B per window-restricted iA id,
-24 B per result; counted by the kernel itself) over the kernel's average
-duration measured with HIP events on its own stream.
+The headline value is the learned+batched form: the batched engine
+(autotuned launch configuration, one hipGraph per batch) with the learned
+e-value surrogate, which is trained first on the GPU from 500k spectra of
+held-out seeds searched with the plain tail fit. The plain tail fit judges
+the learned run (top hits, e-value error) and the plain forms are timed the
+same way beside it: batched with the plain fit, and the per-spectrum engine.
+
+Prints one JSON line. `roofline.achieved` is the search kernel's algorithmic
+bytes (4 B per window-restricted iA id, counted by the kernel itself) over
+its device time measured with HIP events on the context stream; `traffic` is
+the HBM bytes per launch from the committed rocprofv3 counter summary.
 """
 from __future__ import annotations
 
@@ -50,8 +57,9 @@ def algorithmic_bytes(st: dict, n_peaks: int) -> int:
 
 
 def traffic_from_profiles(workload: str):
-    """HBM bytes per launch from the committed rocprofv3 PMC summary, if any."""
-    f = ROOT / "profiles" / f"search_kernel_{workload}_hbm_traffic.txt"
+    """HBM bytes per launch of the search kernel from the committed rocprofv3
+    PMC summary (FETCH_SIZE and WRITE_SIZE, scaled by a copy calibration)."""
+    f = ROOT / "profiles" / f"search_items_{workload}_hbm_traffic.txt"
     if not f.exists():
         return None
     for line in f.read_text().splitlines():
@@ -81,6 +89,67 @@ def cpu_baseline(index, batch, params_ffi, budget_s: float):
     dt = time.perf_counter() - t
     return {"value": round(m / dt, 2), "unit": "spectra/s", "cores": cores, "kind": "port",
             "sample": f"first {m} spectra of the same batch, OpenMP over spectra, {dt:.1f} s"}
+
+
+def train_surrogate(s, index, sc, dM: float, args) -> dict:
+    """Train the learned e-value model on the GPU from (histogram, plain-fit)
+    pairs of spectra generated with held-out seeds (not the timed batch)."""
+    import dataclasses
+
+    from gicops_amd import slm
+    from gicops_amd.search import SearchParams
+    from gicops_amd.surrogate import Surrogate, TrainParams
+
+    t = time.perf_counter()
+    sur = Surrogate(s, seed=7)
+    plain = SearchParams(dM=dM)
+    left, seed, rows = args.train_spectra, 1000, 0
+    while left > 0:
+        n = min(left, 100_000)
+        b = slm.generate_spectra(index, dataclasses.replace(sc, n_spectra=n, seed=seed))
+        dq = s.upload(b)
+        s.search_device(dq, plain)
+        rows += sur.add_training(dq)
+        dq.free()
+        left -= n
+        seed += 1
+    t_data = time.perf_counter() - t
+    t = time.perf_counter()
+    rep = sur.train(TrainParams(batch=1024, epochs=args.train_epochs, lr=2e-3, lr_decay=0.7, conf_weight=0.5,
+                                tol=1e-3, seed=1))
+    t_train = time.perf_counter() - t
+    sur.attach()
+    log(f"surrogate: {rows} training pairs from {args.train_spectra} spectra ({t_data:.1f} s), trained "
+        f"{rep['epochs']} epochs in {t_train:.1f} s; mse_u {rep['mse_u'][0]:.4f} -> {rep['mse_u'][-1]:.4f}, "
+        f"within tol {rep['within_tol'][-1]:.4f}")
+    return {"train_pairs": rows, "train_spectra": args.train_spectra, "epochs": rep["epochs"],
+            "train_s": round(t_train, 2), "data_s": round(t_data, 2),
+            "mse_u_first_last": [round(rep["mse_u"][0], 5), round(rep["mse_u"][-1], 5)],
+            "train_within_tol_last": round(rep["within_tol"][-1], 5), "conf_threshold": args.surrogate_conf}
+
+
+def judge_learned(psm, ref, st) -> dict:
+    """The plain tail fit as the judge of the learned run: top hits must be
+    identical; e-value error in relative log10 over the modeled spectra."""
+    from gicops_amd import _ffi as F
+
+    same = all(np.array_equal(psm[k], ref[k]) for k in ("psid", "idxoffset", "hyperscore", "cpsms", "hyp",
+                                                        "sharedions"))
+    modeled = (ref["flags"] & F.GX_PSM_MODELED) != 0
+    sur = (psm["flags"] & F.GX_PSM_SURROGATE) != 0
+    fit = (ref["flags"] & F.GX_PSM_FIT_OK) != 0
+    ok = np.zeros(len(psm), bool)
+    with np.errstate(divide="ignore", invalid="ignore"):
+        lr = np.log10(ref["evalue"][fit])
+        ll = np.log10(psm["evalue"][fit])
+    ok[fit] = np.abs(ll - lr) <= 1e-3 * np.maximum(np.abs(lr), 1e-3)
+    ok[~fit] = ~sur[~fit]
+    err = np.abs(ll - lr)[sur[fit]] if sur.any() else np.zeros(0)
+    return {"top_hits_identical": bool(same), "modeled": int(modeled.sum()), "surrogate_served": int(sur.sum()),
+            "coverage": round(float(sur.sum()) / max(1, int(modeled.sum())), 5),
+            "fallbacks": int(st["fallbacks"]),
+            "within_1e-3_rel_log10": round(float(ok[modeled].mean()), 5),
+            "served_median_abs_dlog10e": round(float(np.median(err)), 5) if len(err) else None}
 
 
 def calibrate_copy(mib: int) -> int:
@@ -116,7 +185,13 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--engine", default="batched", choices=["batched", "spectrum"])
     ap.add_argument("--config", type=int, default=-1, help="batched launch configuration (-1: try all, keep the best)")
-    ap.add_argument("--compare-plain", action="store_true", help="also time the per-spectrum kernel once")
+    ap.add_argument("--no-compare-spectrum", action="store_true",
+                    help="skip the one run of the per-spectrum engine with the plain fit")
+    ap.add_argument("--evalue", default="learned", choices=["learned", "plain"])
+    ap.add_argument("--train-spectra", type=int, default=500_000,
+                    help="spectra searched (held-out seeds) to make surrogate training pairs")
+    ap.add_argument("--train-epochs", type=int, default=6)
+    ap.add_argument("--surrogate-conf", type=float, default=0.9)
     ap.add_argument("--no-graph", action="store_true", help="time eager launches instead of hipGraph replays")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run a device-to-device copy of MIB MiB (HBM counter calibration) and exit")
@@ -166,34 +241,61 @@ def main() -> int:
             f"{tune['split']}; tried " + ", ".join(f"c{c}/s{sp}: {ms:.1f} ms" for c, sp, ms in tune["tried"]))
         extra["autotune"] = {"config": tune["config"], "split": tune["split"],
                              "tried_ms": {f"c{c}s{sp}": round(ms, 3) for c, sp, ms in tune["tried"]}}
-    # one eager run for the per-kernel device times (HIP events), then graph replays
-    st = s.search_device(dq, params)
-    log(f"eager: total {st['total_ms']:.1f} ms (plan {st['plan_ms']:.1f}, search {st['kernel_ms']:.1f}, "
+    # one eager run of the plain form for the per-kernel device times (HIP
+    # events); its results are also the judge of the learned e-values
+    plain = SearchParams(dM=dM, engine=engine, config=args.config)
+    st = s.search_device(dq, plain)
+    log(f"eager plain: total {st['total_ms']:.1f} ms (plan {st['plan_ms']:.1f}, search {st['kernel_ms']:.1f}, "
         f"evalue {st['evalue_ms']:.1f}), {st['items']} items, {st['chunks']} chunks, config {st['config']}")
     eager = dict(st)
-    if engine == F.GX_ENGINE_BATCHED and not args.no_graph:
-        params.use_graph = True
-    if args.compare_plain:
+    judge, _ = s.download(dq)
+    use_graph = engine == F.GX_ENGINE_BATCHED and not args.no_graph
+    learned = args.evalue == "learned" and engine == F.GX_ENGINE_BATCHED
+    if learned:
+        extra["surrogate"] = train_surrogate(s, index, sc, dM, args)
+        params = SearchParams(dM=dM, engine=engine, config=args.config, evalue_mode=F.GX_EVALUE_LEARNED,
+                              surrogate_conf=args.surrogate_conf)
+        st = s.search_device(dq, params)
+        log(f"eager learned: total {st['total_ms']:.1f} ms (evalue stage {st['evalue_ms']:.2f} ms, "
+            f"{st['fallbacks']} plain-fit fallbacks)")
+        extra["learned_evalue_ms"] = round(st["evalue_ms"], 3)
+    else:
+        params = plain
+    extra["plain_evalue_ms"] = round(eager["evalue_ms"], 3)
+    params.use_graph = use_graph
+
+    def timed(p, label):
+        for i in range(args.warmup):
+            st = s.search_device(dq, p)
+            log(f"{label} warmup {i}: total {st['total_ms']:.1f} ms")
+        t0 = time.perf_counter()
+        for i in range(args.steps):
+            st = s.search_device(dq, p)
+            log(f"{label} step {i}: total {st['total_ms']:.1f} ms")
+        return (time.perf_counter() - t0) * 1000.0 / args.steps, st
+
+    ms_step, st_last = timed(params, "learned" if learned else "plain")
+    psm, _ = s.download(dq)
+    value = batch.n_spectra / (ms_step / 1000.0)
+    if learned:
+        extra["judge"] = judge_learned(psm, judge, st_last)
+        log(f"judge: {extra['judge']}")
+        # the plain tail-fit forms, timed the same way
+        pp = SearchParams(dM=dM, engine=engine, config=args.config, use_graph=use_graph)
+        ms_plain, _ = timed(pp, "plain")
+        extra["plain_batched_spectra_per_s"] = round(batch.n_spectra / (ms_plain / 1000.0), 1)
+        extra["plain_batched_ms_per_step"] = round(ms_plain, 3)
+    if engine == F.GX_ENGINE_BATCHED and not args.no_compare_spectrum:
         pp = SearchParams(dM=dM, engine=F.GX_ENGINE_SPECTRUM)
         st = s.search_device(dq, pp)
-        extra["per_spectrum_kernel_ms"] = round(st["kernel_ms"], 3)
-        extra["per_spectrum_spectra_per_s"] = round(batch.n_spectra / (st["kernel_ms"] / 1000.0), 1)
-        log(f"per-spectrum kernel: {st['kernel_ms']:.1f} ms")
-    for i in range(args.warmup):
-        st = s.search_device(dq, params)
-        log(f"warmup {i}: total {st['total_ms']:.1f} ms")
-    t0 = time.perf_counter()
-    for i in range(args.steps):
-        st = s.search_device(dq, params)
-        log(f"step {i}: total {st['total_ms']:.1f} ms")
-    t1 = time.perf_counter()
-    psm, _ = s.download(dq)
-    ms_step = (t1 - t0) * 1000.0 / args.steps
-    value = batch.n_spectra / (ms_step / 1000.0)
+        extra["plain_spectrum_engine_ms"] = round(st["total_ms"], 3)
+        extra["plain_spectrum_engine_spectra_per_s"] = round(batch.n_spectra / (st["total_ms"] / 1000.0), 1)
+        log(f"per-spectrum engine with the plain fit: {st['total_ms']:.1f} ms")
     st = eager
     avg_kms = st["kernel_ms"]
     abytes = algorithmic_bytes(st, batch.n_peaks)
-    achieved = 4 * st["ions"] / (avg_kms / 1000.0) / 1e9
+    achieved = 4 * st["ions"] / (avg_kms / 1000.0) / 1e9  # = per-launch bytes / average launch time
+    launches = max(1, int(st["chunks"])) if engine == F.GX_ENGINE_BATCHED else 1
     reported = int(((psm["flags"] & 8) > 0).sum())
     log(f"{value:.0f} spectra/s, kernel {avg_kms:.1f} ms, {achieved:.0f} GB/s algorithmic, "
         f"H={st['ions']} B={st['bins']} W={st['window']} cand={st['candidates']} reported={reported}")
@@ -203,7 +305,9 @@ def main() -> int:
         cpu = cpu_baseline(index, batch, O.default_params(dM=dM), args.cpu_budget)
         log(f"cpu baseline: {cpu}")
 
-    traffic = traffic_from_profiles(args.workload)
+    # the committed counter summary covers the batched kernel in configuration 1
+    traffic = traffic_from_profiles(args.workload) if (engine == F.GX_ENGINE_BATCHED and eager["config"] == 1) \
+        else None
     out = {
         "metric": "spectra/sec, 2M-peptide open search: learned+batched vs plain tail-fit, 1 GPU",
         "value": round(value, 1),
@@ -225,12 +329,11 @@ def main() -> int:
             "spectra": batch.n_spectra,
             "dM_da": dM,
             "dF_bins": params.dF,
-            "evalue": "plain tail fit",
+            "evalue": "learned surrogate, plain tail-fit fallback" if learned else "plain tail fit",
             "engine": args.engine,
             "launch_config": eager["config"],
             "graph": bool(params.use_graph),
             "plan_ms": round(eager["plan_ms"], 3),
-            "evalue_ms": round(eager["evalue_ms"], 3),
             **extra,
         },
         "roofline": {
@@ -240,11 +343,13 @@ def main() -> int:
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
             "traffic": traffic,
-            "algorithmic_bytes_per_launch": abytes,
             "kernel": "search_items" if engine == F.GX_ENGINE_BATCHED else "search_kernel",
-            "kernel_bytes": "4 B per window-restricted iA id (H) in the search kernel",
+            "launches": launches,
+            "avg_launch_ms": round(avg_kms / launches, 4),
+            "algorithmic_bytes_per_launch": round(4 * st["ions"] / launches),
+            "algorithmic_bytes_rule": "4 B per window-restricted iA id (H)",
+            "job_algorithmic_bytes": abytes,
             "job_algorithmic_gbs": round(abytes / (ms_step / 1000.0) / 1e9, 1),
-            "kernel_ms": round(avg_kms, 3),
         },
         "cpu_baseline": cpu,
     }

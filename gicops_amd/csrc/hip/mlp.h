@@ -91,7 +91,9 @@ hipError_t launch_to_bf16(const float *x, bf16 *y, uint64_t n, hipStream_t s);
 hipError_t launch_gather_rows(const bf16 *src, const float *tsrc, const float *lsrc, const uint32_t *idx, uint32_t n,
                               bf16 *dst, float *tdst, float *ldst, hipStream_t s);
 
-// route: accept the surrogate where confident and not sparse
+// route: accept the surrogate where confident, not sparse, in distribution
+// (normalised outputs within kMaxZ) and with a fitted log-survival u <= 0
+constexpr float kMaxZ = 4.0f;
 struct RouteArgs {
     uint32_t n;
     const float *out;       // n * kOutPad

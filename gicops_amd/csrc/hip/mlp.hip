@@ -272,13 +272,17 @@ __global__ void route_kernel(RouteArgs a) {
         if (modeled && !a.sparse[s]) {
             const float *o = a.out + (size_t)s * kOutPad;
             const float p = 1.0f / (1.0f + expf(-o[2]));
-            if (p >= a.conf_threshold) {
+            // outputs beyond kMaxZ standard deviations of the training targets
+            // are out of distribution, whatever the confidence says
+            const bool in_range = fabsf(o[0]) <= kMaxZ && fabsf(o[1]) <= kMaxZ;
+            if (p >= a.conf_threshold && in_range) {
                 gx_psm r = a.psm[s];
                 const double slope = (double)o[0] * a.norm.a_scale + a.norm.a_shift;
                 const double u = (double)a.anchor[s] + (double)o[1] * a.norm.u_scale + a.norm.u_shift;
                 const double e = (double)r.cpsms * pow(10.0, u);
                 const double am = slope * 1e6, bm = (u - slope * r.hyp) * 1e6;
-                if (isfinite(e) && fabs(am) < 2147483647.0 && fabs(bm) < 2147483647.0) {
+                // a survival probability: u <= 0
+                if (u <= 0.0 && isfinite(e) && fabs(am) < 2147483647.0 && fabs(bm) < 2147483647.0) {
                     r.mu = (int32_t)am;
                     r.beta = (int32_t)bm;
                     r.evalue = e;

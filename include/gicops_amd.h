@@ -228,8 +228,10 @@ int gx_ctx_device_info(gx_ctx *ctx, char *name, size_t name_len, int *n_cu);
  * model of expeRT::ModelTailFit (expeRT.cpp:396-559): slope, fitted
  * log10-survival at the top hit's bin, and a confidence logit. With
  * evalue_mode = GX_EVALUE_LEARNED the batched engine takes the surrogate's
- * e-value where the confidence is at least surrogate_conf and the histogram
- * has at least 3 non-empty bins below the top hit; every other modeled
+ * e-value where the confidence is at least surrogate_conf, the histogram
+ * has at least 3 non-empty bins below the top hit, both regression outputs
+ * lie within 4 standard deviations of the training targets and the fitted
+ * log10 survival at the top bin is <= 0; every other modeled
  * spectrum gets the plain tail fit (GX_PSM_SURROGATE tells them apart, and
  * stats.fallbacks counts the plain fits). Top hits, cpsms and histograms are
  * the same in both modes. No reference counterpart: the reference always
