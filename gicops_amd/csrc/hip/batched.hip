@@ -30,6 +30,7 @@ constexpr int kPeakCap = kMaxPeaksLds;  // 2048 legal peaks per spectrum
 constexpr int kStatSpread = kStatRows;
 constexpr int kUnroll = 4;              // pieces in flight per wave
 constexpr uint32_t kWholeBin = 64;      // bins this small skip the boundary search
+constexpr int kItemCandLds = 768;       // candidates kept in LDS per item (the rest in global scratch)
 constexpr uint32_t kWholeFlag = 0x80000000u;  // marks a whole-bin slice start (ion positions < 2^31)
 
 __device__ __forceinline__ void stat_add(unsigned long long *stats, int which, unsigned long long v) {
@@ -225,7 +226,7 @@ __global__ void __launch_bounds__(kPlanThreads) plan_slices_kernel(PlanArgs a) {
     for (int j = tid; j < nsl; j += kPlanThreads) {
         const int k = j / nb5;
         const uint32_t bin = (uint32_t)(lq[k] - dF + (j - k * nb5));
-        const uint32_t st = bk.bA[bin], en = bk.bA[bin + 1];
+        const uint32_t st = gptr(bk.bA)[bin], en = gptr(bk.bA)[bin + 1];
         uint32_t *out = a.pos + base + (uint64_t)j * stride;
         // bins of at most kWholeBin ions are streamed whole by the search
         // kernel (flagged start, every tile filters by id); larger ones get
@@ -240,7 +241,7 @@ __global__ void __launch_bounds__(kPlanThreads) plan_slices_kernel(PlanArgs a) {
         uint32_t cur = st;
         for (uint32_t t = 0; t <= tiles; t++) {
             const uint32_t id = (t == tiles) ? end_id : (lo + t * T) * speclen;
-            if (en > cur) cur += lower_bound_u32(bk.iA + cur, en - cur, id);
+            if (en > cur) cur += lower_bound_u32(gptr(bk.iA) + cur, en - cur, id);
             out[t] = cur;
         }
         bins++;
@@ -266,8 +267,9 @@ struct ItemLds {
     uint32_t s_flat[SL + 1];   // exclusive prefix of their lengths
     uint32_t wsum2[THREADS / 64];
     uint16_t cj0[THREADS];     // slice of each chunk's first id, per round
+    unsigned long long cmask[THREADS];  // lanes of each chunk where a later slice starts
     uint32_t ne;
-    uint32_t cand[kCandLds];
+    uint32_t cand[kItemCandLds];
     uint32_t hist[GX_HIST_BINS];
     uint32_t wsum[THREADS / 64];
     unsigned long long w_key[THREADS / 64];
@@ -363,7 +365,8 @@ __global__ void __launch_bounds__(THREADS) search_items_kernel(ItemArgs a) {
         const DevBucket bk = a.buckets[b];
         const uint32_t speclen = bk.speclen;
         const uint32_t tlo_id = (uint32_t)tlo * speclen;
-        const uint32_t thi_id = (uint32_t)(thi + 1) * speclen;
+        const uint32_t span_id = nent * speclen;
+        const uint32_t fsh = 5u - lp;  // log2 of the counter width
         const int P = (int)a.n_legal[s];
         const uint32_t qoff = a.qidx[s];
         for (int k = tid; k < P; k += THREADS) {
@@ -373,9 +376,10 @@ __global__ void __launch_bounds__(THREADS) search_items_kernel(ItemArgs a) {
         const int nsl = P * nb5;
         const uint32_t stride = tiles + 1;
         const uint32_t *pos = a.pos + (a.pos_off[s] - a.pos_off[a.chunk_s0]) + a.pair_pos_off[pi];
-        const uint32_t *ia = bk.iA;
+        const auto ia = gptr(bk.iA);
         // ---- pass 1: shared-peak counts of every entry in the tile ----
         const uint32_t dm = bk.div_m, dsh = bk.div_s;
+        uint32_t n_scanned = 0, n_errors = 0;
         for (int c0 = 0; c0 < nsl; c0 += SL) {
             const int ns = min(SL, nsl - c0);
             // slice ranges of this tile; non-empty ones compacted in order
@@ -414,60 +418,65 @@ __global__ void __launch_bounds__(THREADS) search_items_kernel(ItemArgs a) {
             }
             __syncthreads();
             // the tile's ions, flattened over the non-empty slices, in rounds
-            // of THREADS 64-id chunks. The slice holding each chunk's first id
-            // is found first (one thread per chunk); within a chunk, the slices
-            // that start inside it form a bit mask built with scalar readlanes,
-            // and lane L's slice is the chunk's first plus the starts <= L.
+            // of THREADS 64-id chunks. One thread per chunk finds the slice
+            // holding the chunk's first id and a bit mask of the lanes where
+            // later slices start.
             const uint32_t ne = L.ne;
             const uint32_t N = L.s_flat[ne];
             const uint32_t nch = (N + 63) >> 6;
             for (uint32_t r0 = 0; r0 < nch; r0 += THREADS) {
                 const uint32_t nr = min((uint32_t)THREADS, nch - r0);
-                if ((uint32_t)tid < nr)
-                    L.cj0[tid] = (uint16_t)(upper_bound_lds(L.s_flat, (int)ne + 1, (r0 + tid) << 6) - 1);
+                if ((uint32_t)tid < nr) {
+                    const uint32_t f0 = (r0 + tid) << 6;
+                    const uint32_t j = (uint32_t)upper_bound_lds(L.s_flat, (int)ne + 1, f0) - 1;
+                    unsigned long long mk = 0;
+                    for (uint32_t k = j + 1; k < ne; k++) {
+                        const uint32_t d = L.s_flat[k] - f0;
+                        if (d >= 64) break;
+                        mk |= 1ull << d;
+                    }
+                    L.cj0[tid] = (uint16_t)j;
+                    L.cmask[tid] = mk;
+                }
                 __syncthreads();
+                // U chunks per wave in three batched LDS/HBM stages: chunk
+                // metadata, slice bases, ion ids; lane L's slice is the chunk's
+                // first plus the slice starts at lanes <= L
                 constexpr int U = 2 * kUnroll;
+                const unsigned long long upto = lane == 63 ? ~0ull : ((1ull << (uint32_t)(lane + 1)) - 1ull);
                 for (uint32_t c = (uint32_t)wave * U; c < nr; c += NW * U) {
-                    uint32_t raw[U];
-                    bool ok[U];
+                    uint32_t j0[U], sa[U], sf[U], raw[U];
+                    unsigned long long mk[U];
 #pragma unroll
                     for (int u = 0; u < U; u++) {
-                        ok[u] = false;
-                        raw[u] = 0;
-                        const uint32_t cc = c + u;
-                        if (cc < nr) {
-                            const uint32_t f0 = (r0 + cc) << 6;
-                            const uint32_t f = f0 + (uint32_t)lane;
-                            const uint32_t j0 = L.cj0[cc];
-                            const uint32_t k = j0 + 1 + (uint32_t)lane;
-                            uint32_t d = 64;
-                            if (k < ne) d = L.s_flat[k] - f0;
-                            const unsigned long long inside = __ballot(d < 64);
-                            uint32_t j = j0;
-                            if (inside) {
-                                unsigned long long starts = 0;
-                                const int m = __popcll(inside);  // slices j0+1 .. j0+m start here
-                                for (int i = 0; i < m; i++)
-                                    starts |= 1ull << __builtin_amdgcn_readlane(d, i);
-                                const unsigned long long upto =
-                                    lane == 63 ? ~0ull : ((1ull << (uint32_t)(lane + 1)) - 1ull);
-                                j = j0 + (uint32_t)__popcll(starts & upto);
-                            }
-                            if (f < N) {
-                                ok[u] = true;
-                                raw[u] = ia[L.s_a[j] + (f - L.s_flat[j])];
-                            }
-                        }
+                        const uint32_t cc = min(c + (uint32_t)u, nr - 1);
+                        j0[u] = L.cj0[cc];
+                        mk[u] = L.cmask[cc];
                     }
 #pragma unroll
                     for (int u = 0; u < U; u++) {
-                        if (ok[u] && raw[u] >= tlo_id && raw[u] < thi_id) {
+                        const uint32_t j = j0[u] + (uint32_t)__popcll(mk[u] & upto);
+                        sa[u] = L.s_a[j];
+                        sf[u] = L.s_flat[j];
+                    }
+#pragma unroll
+                    for (int u = 0; u < U; u++) {
+                        const uint32_t f = ((r0 + c + (uint32_t)u) << 6) + (uint32_t)lane;
+                        const bool ok = c + (uint32_t)u < nr && f < N;
+                        // a failed lane gets an id just below the tile: filtered below
+                        raw[u] = ok ? ia[sa[u] + (f - sf[u])] : tlo_id - 1u;
+                    }
+#pragma unroll
+                    for (int u = 0; u < U; u++) {
+                        // ids of the tile's entries: [tlo_id, tlo_id + span_id), so
+                        // el < nent follows; the guard only keeps LDS writes in the tile
+                        if (raw[u] - tlo_id < span_id) {
                             const uint32_t el = (__umulhi(raw[u], dm) >> dsh) - (uint32_t)tlo;
                             if (el < nent) {
-                                atomicAdd(&L.cnt[el >> lp], 1u << ((el & (pack - 1)) * fbits));
-                                my_scanned++;
+                                atomicAdd(&L.cnt[el >> lp], 1u << ((el & (pack - 1)) << fsh));
+                                n_scanned++;
                             } else {
-                                my_errors++;  // never expected: an id outside the tile
+                                n_errors++;  // never expected (exact division is checked on the host)
                             }
                         }
                     }
@@ -476,6 +485,8 @@ __global__ void __launch_bounds__(THREADS) search_items_kernel(ItemArgs a) {
             }
             __syncthreads();
         }
+        my_scanned += n_scanned;
+        my_errors += n_errors;
         // ---- candidates: entries with count >= min_shp; clear the tile ----
         if (tid == 0) L.n_cand = 0;
         __syncthreads();
@@ -506,10 +517,10 @@ __global__ void __launch_bounds__(THREADS) search_items_kernel(ItemArgs a) {
                         if (cc >= min_cnt && el < nent) {
                             const uint32_t p = atomicAdd(&L.n_cand, 1u);
                             const uint32_t psid = (uint32_t)tlo + el;
-                            if (p < (uint32_t)kCandLds)
+                            if (p < (uint32_t)kItemCandLds)
                                 L.cand[p] = psid;
                             else
-                                g_cand[p - kCandLds] = psid;
+                                g_cand[p - kItemCandLds] = psid;
                         }
                     }
                 }
@@ -530,8 +541,8 @@ __global__ void __launch_bounds__(THREADS) search_items_kernel(ItemArgs a) {
                 const uint32_t c = __umulhi(tk, bk.div_m) >> bk.div_s;
                 const uint32_t r = tk - c * speclen;
                 const uint32_t ci = c0 + c;
-                const uint32_t psid = ci < (uint32_t)kCandLds ? L.cand[ci] : g_cand[ci - kCandLds];
-                const int32_t x = (int32_t)bk.ion_bin[(size_t)psid * speclen + r];
+                const uint32_t psid = ci < (uint32_t)kItemCandLds ? L.cand[ci] : gptr(g_cand)[ci - kItemCandLds];
+                const int32_t x = (int32_t)gptr(bk.ion_bin)[(size_t)psid * speclen + r];
                 int k = lower_bound_lds(L.pk_q, P, x - dF);
                 uint32_t n = 0, sum = 0;
                 for (; k < P && L.pk_q[k] <= x + dF; k++) {
@@ -553,7 +564,7 @@ __global__ void __launch_bounds__(THREADS) search_items_kernel(ItemArgs a) {
                 acc_ib[c] = 0;
                 acc_iy[c] = 0;
                 const uint32_t ci = c0 + c;
-                const uint32_t psid = ci < (uint32_t)kCandLds ? L.cand[ci] : g_cand[ci - kCandLds];
+                const uint32_t psid = ci < (uint32_t)kItemCandLds ? L.cand[ci] : gptr(g_cand)[ci - kItemCandLds];
                 const uint32_t shpk = bc + yc;
                 if (shpk >= (uint32_t)a.min_shp && (int)bc < a.lgfact_n && (int)yc < a.lgfact_n) {
                     float hs = hyperscore(bc, yc, ibc, iyc, a.lgfact);

@@ -11,7 +11,20 @@
 
 namespace gx {
 
-__device__ __forceinline__ uint32_t lower_bound_u32(const uint32_t *a, uint32_t n, uint32_t v) {
+// Index arrays reached through a pointer loaded from memory (DevBucket) are
+// generic pointers to the compiler, whose loads become FLAT loads; those also
+// count in the LDS counter, so every LDS wait then waits for them too. These
+// views make them global loads (vector memory counter only).
+template <class T>
+using gmem = const __attribute__((address_space(1))) T;
+
+template <class T>
+__device__ __forceinline__ gmem<T> *gptr(const T *p) {
+    return (gmem<T> *)p;
+}
+
+template <class P>
+__device__ __forceinline__ uint32_t lower_bound_u32(P a, uint32_t n, uint32_t v) {
     uint32_t lo = 0;
     while (n > 0) {
         uint32_t half = n >> 1;
@@ -25,7 +38,8 @@ __device__ __forceinline__ uint32_t lower_bound_u32(const uint32_t *a, uint32_t 
     return lo;
 }
 
-__device__ __forceinline__ uint32_t lower_bound_f32(const float *a, uint32_t n, float v) {
+template <class P>
+__device__ __forceinline__ uint32_t lower_bound_f32(P a, uint32_t n, float v) {
     uint32_t lo = 0;
     while (n > 0) {
         uint32_t half = n >> 1;
@@ -39,7 +53,8 @@ __device__ __forceinline__ uint32_t lower_bound_f32(const float *a, uint32_t n, 
     return lo;
 }
 
-__device__ __forceinline__ uint32_t upper_bound_f32(const float *a, uint32_t n, float v) {
+template <class P>
+__device__ __forceinline__ uint32_t upper_bound_f32(P a, uint32_t n, float v) {
     uint32_t lo = 0;
     while (n > 0) {
         uint32_t half = n >> 1;
@@ -150,20 +165,21 @@ __device__ __forceinline__ bool precursor_window(const DevBucket &bk, float prec
     if (n == 0 || dM < 0.0) return false;
     const float pm1 = (float)((double)prec - dM);
     const float pm2 = (float)((double)prec + dM);
-    const float m0 = bk.mass[0], mN = bk.mass[n - 1];
+    const auto mass = gptr(bk.mass);
+    const float m0 = mass[0], mN = mass[n - 1];
     if (pm1 < m0)
         lo = 0;
     else if (pm1 > mN)
         return false;
     else
-        lo = (int32_t)lower_bound_f32(bk.mass, n, pm1);
+        lo = (int32_t)lower_bound_f32(mass, n, pm1);
     if (pm2 > mN)
         hi = (int32_t)n - 1;
     else if (pm2 < m0)
         return false;
     else
-        hi = (int32_t)upper_bound_f32(bk.mass, n, pm2) - 1;
-    const bool ok = hi >= 0 && lo < (int32_t)n && bk.mass[hi] <= pm2 && bk.mass[lo] >= pm1 && hi >= lo;
+        hi = (int32_t)upper_bound_f32(mass, n, pm2) - 1;
+    const bool ok = hi >= 0 && lo < (int32_t)n && mass[hi] <= pm2 && mass[lo] >= pm1 && hi >= lo;
     if (!ok) hi = lo - 1;
     return ok;
 }

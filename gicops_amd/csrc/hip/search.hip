@@ -205,9 +205,9 @@ __global__ void __launch_bounds__(kSearchThreads, 4) search_kernel(SearchArgs a)
             for (int j = tid; j < nsl; j += kSearchThreads) {
                 const int k = j / nbin5;
                 const uint32_t bin = (uint32_t)(L.pk_q[k] - dF + (j - k * nbin5));
-                const uint32_t st = bk.bA[bin], en = bk.bA[bin + 1];
+                const uint32_t st = gptr(bk.bA)[bin], en = gptr(bk.bA)[bin + 1];
                 uint32_t c = en;
-                if (en > st) c = st + lower_bound_u32(bk.iA + st, en - st, lo_id);
+                if (en > st) c = st + lower_bound_u32(gptr(bk.iA) + st, en - st, lo_id);
                 sl_cur[j] = c;
                 sl_end[j] = en;
                 my_bins++;
@@ -223,7 +223,7 @@ __global__ void __launch_bounds__(kSearchThreads, 4) search_kernel(SearchArgs a)
                     const uint32_t e = sl_end[j];
                     if (e - c > kLight || c >= e) continue;
                     while (c < e) {
-                        const uint32_t raw = bk.iA[c];
+                        const uint32_t raw = gptr(bk.iA)[c];
                         if (raw >= id_end) break;
                         const uint32_t el = raw / speclen - tbase;
                         atomicAdd(&L.cnt[el >> 1], 1u << ((el & 1u) << 4));
@@ -241,7 +241,7 @@ __global__ void __launch_bounds__(kSearchThreads, 4) search_kernel(SearchArgs a)
                     for (;;) {
                         const uint32_t idx = c + (uint32_t)lane;
                         const bool inb = idx < e;
-                        const uint32_t raw = inb ? bk.iA[idx] : 0xffffffffu;
+                        const uint32_t raw = inb ? gptr(bk.iA)[idx] : 0xffffffffu;
                         const bool valid = inb && raw < id_end;
                         if (valid) {
                             const uint32_t el = raw / speclen - tbase;
@@ -282,7 +282,7 @@ __global__ void __launch_bounds__(kSearchThreads, 4) search_kernel(SearchArgs a)
                 const uint32_t half = speclen >> 1;
                 for (uint32_t ci = wave; ci < ncand; ci += kWaves) {
                     const uint32_t psid = ci < kCandLds ? L.cand[ci] : g_cand[ci - kCandLds];
-                    const uint32_t *bins = bk.ion_bin + (size_t)psid * speclen;
+                    const auto bins = gptr(bk.ion_bin) + (size_t)psid * speclen;
                     uint32_t bc = 0, yc = 0, ibc = 0, iyc = 0;
                     for (uint32_t r = lane; r < speclen; r += 64) {
                         const int32_t x = (int32_t)bins[r];
