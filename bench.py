@@ -192,6 +192,7 @@ def main() -> int:
                     help="spectra searched (held-out seeds) to make surrogate training pairs")
     ap.add_argument("--train-epochs", type=int, default=6)
     ap.add_argument("--surrogate-conf", type=float, default=0.9)
+    ap.add_argument("--whole-bin", type=int, default=0, help="whole-bin threshold (0: tuned)")
     ap.add_argument("--no-graph", action="store_true", help="time eager launches instead of hipGraph replays")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run a device-to-device copy of MIB MiB (HBM counter calibration) and exit")
@@ -225,7 +226,7 @@ def main() -> int:
     log(f"device: {name}, {ncu} CUs")
     from gicops_amd import _ffi as F
     engine = F.GX_ENGINE_BATCHED if args.engine == "batched" else F.GX_ENGINE_SPECTRUM
-    params = SearchParams(dM=dM, engine=engine, config=args.config)
+    params = SearchParams(dM=dM, engine=engine, config=args.config, whole_bin=args.whole_bin)
     t = time.perf_counter()
     dq = s.upload(batch)
     log(f"query upload {time.perf_counter() - t:.2f} s")
@@ -238,12 +239,13 @@ def main() -> int:
         tune = s.autotune(sample, params)
         sample.free()
         log(f"autotune on {ns} spectra ({time.perf_counter() - t:.1f} s): config {tune['config']}, split "
-            f"{tune['split']}; tried " + ", ".join(f"c{c}/s{sp}: {ms:.1f} ms" for c, sp, ms in tune["tried"]))
-        extra["autotune"] = {"config": tune["config"], "split": tune["split"],
-                             "tried_ms": {f"c{c}s{sp}": round(ms, 3) for c, sp, ms in tune["tried"]}}
+            f"{tune['split']}, whole bins <= {tune['whole_bin']}; tried "
+            + ", ".join(f"c{c}/s{sp}/w{w}: {ms:.1f} ms" for c, sp, w, ms in tune["tried"]))
+        extra["autotune"] = {"config": tune["config"], "split": tune["split"], "whole_bin": tune["whole_bin"],
+                             "tried_ms": {f"c{c}s{sp}w{w}": round(ms, 3) for c, sp, w, ms in tune["tried"]}}
     # one eager run of the plain form for the per-kernel device times (HIP
     # events); its results are also the judge of the learned e-values
-    plain = SearchParams(dM=dM, engine=engine, config=args.config)
+    plain = SearchParams(dM=dM, engine=engine, config=args.config, whole_bin=args.whole_bin)
     st = s.search_device(dq, plain)
     log(f"eager plain: total {st['total_ms']:.1f} ms (plan {st['plan_ms']:.1f}, search {st['kernel_ms']:.1f}, "
         f"evalue {st['evalue_ms']:.1f}), {st['items']} items, {st['chunks']} chunks, config {st['config']}")
@@ -254,7 +256,7 @@ def main() -> int:
     if learned:
         extra["surrogate"] = train_surrogate(s, index, sc, dM, args)
         params = SearchParams(dM=dM, engine=engine, config=args.config, evalue_mode=F.GX_EVALUE_LEARNED,
-                              surrogate_conf=args.surrogate_conf)
+                              surrogate_conf=args.surrogate_conf, whole_bin=args.whole_bin)
         st = s.search_device(dq, params)
         log(f"eager learned: total {st['total_ms']:.1f} ms (evalue stage {st['evalue_ms']:.2f} ms, "
             f"{st['fallbacks']} plain-fit fallbacks)")
@@ -281,7 +283,7 @@ def main() -> int:
         extra["judge"] = judge_learned(psm, judge, st_last)
         log(f"judge: {extra['judge']}")
         # the plain tail-fit forms, timed the same way
-        pp = SearchParams(dM=dM, engine=engine, config=args.config, use_graph=use_graph)
+        pp = SearchParams(dM=dM, engine=engine, config=args.config, use_graph=use_graph, whole_bin=args.whole_bin)
         ms_plain, _ = timed(pp, "plain")
         extra["plain_batched_spectra_per_s"] = round(batch.n_spectra / (ms_plain / 1000.0), 1)
         extra["plain_batched_ms_per_step"] = round(ms_plain, 3)

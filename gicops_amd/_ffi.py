@@ -52,7 +52,7 @@ class GxError(RuntimeError):
 def check(func: str, code: int) -> None:
     if code != GX_SUCCESS:
         err = GxError(func, code)
-        if code == GX_ERR_DEVICE and _HIP is not None:
+        if code in (GX_ERR_DEVICE, GX_ERR_INVLD_PARAM, GX_ERR_NOT_READY, GX_ERR_INVLD_INDEX) and _HIP is not None:
             buf = C.create_string_buffer(512)
             _HIP.gx_last_error(buf, 512)
             err = GxError(f"{func} [{buf.value.decode(errors='replace')}]", code)
@@ -81,7 +81,7 @@ class gx_search_params(C.Structure):
         ("dF", C.c_int32), ("scale", C.c_int32), ("max_mass", C.c_int32), ("min_shp", C.c_int32),
         ("min_cpsm", C.c_int32), ("evalue_mode", C.c_int32), ("dM", C.c_double),
         ("expect_max", C.c_double), ("surrogate_conf", C.c_float), ("engine", C.c_int32),
-        ("config", C.c_int32), ("use_graph", C.c_int32), ("split", C.c_int32), ("reserved", C.c_int32),
+        ("config", C.c_int32), ("use_graph", C.c_int32), ("split", C.c_int32), ("whole_bin", C.c_int32),
     ]
 
 
@@ -99,9 +99,9 @@ GX_TUNE_MAX = 16
 
 class gx_tune_result(C.Structure):
     _fields_ = [
-        ("config", C.c_int32), ("split", C.c_int32), ("n_tried", C.c_int32),
+        ("config", C.c_int32), ("split", C.c_int32), ("whole_bin", C.c_int32), ("n_tried", C.c_int32),
         ("tried_config", C.c_int32 * GX_TUNE_MAX), ("tried_split", C.c_int32 * GX_TUNE_MAX),
-        ("tried_ms", C.c_double * GX_TUNE_MAX),
+        ("tried_whole_bin", C.c_int32 * GX_TUNE_MAX), ("tried_ms", C.c_double * GX_TUNE_MAX),
     ]
 
 

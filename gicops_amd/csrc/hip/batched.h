@@ -31,6 +31,7 @@ struct PlanArgs {
     int nbins;
     double dM;
     int tile_words;          // count-tile words per item (the configuration's tile / split)
+    uint32_t whole_bin;      // bins of at most this many ions are streamed whole
     // outputs of plan_spectra
     int32_t *lq;        // sorted legal peaks (CSR at qidx offsets)
     int32_t *li;

@@ -29,7 +29,6 @@ constexpr int kPlanThreads = 256;
 constexpr int kPeakCap = kMaxPeaksLds;  // 2048 legal peaks per spectrum
 constexpr int kStatSpread = kStatRows;
 constexpr int kUnroll = 4;              // pieces in flight per wave
-constexpr uint32_t kWholeBin = 64;      // bins this small skip the boundary search
 constexpr int kItemCandLds = 768;       // candidates kept in LDS per item (the rest in global scratch)
 constexpr uint32_t kWholeFlag = 0x80000000u;  // marks a whole-bin slice start (ion positions < 2^31)
 
@@ -199,13 +198,15 @@ __global__ void __launch_bounds__(1024) scan_kernel(PlanArgs a) {
     }
 }
 
-// one block per (spectrum, bucket) of the chunk
+// one block per (bucket, spectrum) of the chunk, bucket-major: the blocks in
+// flight read one bucket's bA and iA, which stay in L2
 __global__ void __launch_bounds__(kPlanThreads) plan_slices_kernel(PlanArgs a) {
     __shared__ unsigned long long s_st[2];
     const int tid = threadIdx.x, lane = tid & 63;
     const uint32_t nb = (uint32_t)a.n_buckets;
-    const uint32_t s = a.chunk_s0 + blockIdx.x / nb;
-    const uint32_t b = blockIdx.x % nb;
+    const uint32_t nspec = a.chunk_s1 - a.chunk_s0;
+    const uint32_t s = a.chunk_s0 + blockIdx.x % nspec;
+    const uint32_t b = blockIdx.x / nspec;
     const uint32_t pi = s * nb + b;
     const uint32_t tiles = a.pair_tiles[pi];
     if (tiles == 0) return;
@@ -228,11 +229,11 @@ __global__ void __launch_bounds__(kPlanThreads) plan_slices_kernel(PlanArgs a) {
         const uint32_t bin = (uint32_t)(lq[k] - dF + (j - k * nb5));
         const uint32_t st = gptr(bk.bA)[bin], en = gptr(bk.bA)[bin + 1];
         uint32_t *out = a.pos + base + (uint64_t)j * stride;
-        // bins of at most kWholeBin ions are streamed whole by the search
+        // bins of at most whole_bin ions are streamed whole by the search
         // kernel (flagged start, every tile filters by id); larger ones get
         // boundary t = first ion of the slice with id >= (lo + t*T)*speclen,
         // the last one being the window end
-        if (en - st <= kWholeBin) {
+        if (en - st <= a.whole_bin) {
             out[0] = st | kWholeFlag;
             out[1] = en;
             bins++;

@@ -19,6 +19,11 @@ thread_local char g_last_error[512] = "";
 void note_error(const char *what, int line, hipError_t e) {
     std::snprintf(g_last_error, sizeof(g_last_error), "%s (line %d): %s", what, line, hipGetErrorString(e));
 }
+
+int param_error(int code, const char *what) {
+    std::snprintf(g_last_error, sizeof(g_last_error), "invalid argument: %s", what);
+    return code;
+}
 }  // namespace gx
 
 namespace {
@@ -74,10 +79,14 @@ bool make_divisor(uint32_t d, uint32_t &m, uint32_t &s) {
 }
 
 int check_params(const gx_search_params *p, const gx_dindex *idx) {
-    if (p->dF < 0 || p->dF > 64 || p->scale < 1 || p->max_mass < 1 || p->min_shp < 0 || p->min_cpsm < 0)
-        return GX_ERR_INVLD_PARAM;
-    if ((uint64_t)p->max_mass * (uint64_t)p->scale != idx->nbins) return GX_ERR_INVLD_INDEX;
-    if (p->evalue_mode != GX_EVALUE_TAILFIT && p->evalue_mode != GX_EVALUE_LEARNED) return GX_ERR_INVLD_PARAM;
+    if (p->dF < 0 || p->dF > 64) return param_error(GX_ERR_INVLD_PARAM, "dF outside 0..64");
+    if (p->scale < 1 || p->max_mass < 1) return param_error(GX_ERR_INVLD_PARAM, "scale and max_mass must be >= 1");
+    if (p->min_shp < 0 || p->min_cpsm < 0) return param_error(GX_ERR_INVLD_PARAM, "min_shp and min_cpsm must be >= 0");
+    if ((uint64_t)p->max_mass * (uint64_t)p->scale != idx->nbins)
+        return param_error(GX_ERR_INVLD_INDEX, "max_mass * scale differs from the index's bin count");
+    if (p->evalue_mode != GX_EVALUE_TAILFIT && p->evalue_mode != GX_EVALUE_LEARNED)
+        return param_error(GX_ERR_INVLD_PARAM, "unknown evalue_mode");
+    if (p->whole_bin < 0 || p->whole_bin > 4096) return param_error(GX_ERR_INVLD_PARAM, "whole_bin outside 0..4096");
     return GX_SUCCESS;
 }
 
@@ -375,7 +384,7 @@ int search_plain(gx_ctx *c, const gx_dindex *idx, gx_dqueries *q, const gx_searc
 // size the per-spectrum workspace of the batched engine
 int ensure_batched(gx_dqueries *q, const gx_dindex *idx) {
     if (q->lq && q->nb == idx->n_buckets) return GX_SUCCESS;
-    if (q->lq) return GX_ERR_INVLD_PARAM;  // a batch serves one index shape
+    if (q->lq) return param_error(GX_ERR_INVLD_PARAM, "a batch serves one index shape");
     const size_t ns = q->n_spectra, np = q->n_peaks, npairs = ns * (size_t)idx->n_buckets;
     q->nb = idx->n_buckets;
     if (dalloc(&q->lq, np) || dalloc(&q->li, np) || dalloc(&q->n_legal, ns) || dalloc(&q->pair_lo, npairs) ||
@@ -434,6 +443,7 @@ Batched setup_batched(gx_ctx *c, const gx_dindex *idx, gx_dqueries *q, const gx_
     pa.nbins = (int)idx->nbins;
     pa.dM = p->dM;
     pa.tile_words = B.cfg.tile_words / B.split;
+    pa.whole_bin = (uint32_t)(p->whole_bin > 0 ? p->whole_bin : c->tuned_whole);
     pa.lq = q->lq;
     pa.li = q->li;
     pa.n_legal = q->n_legal;
@@ -541,6 +551,7 @@ std::vector<char> graph_key_of(const Batched &B, const gx_search_params *p) {
     gx_search_params pp = *p;
     pp.config = B.cfg_id;
     pp.split = B.split;
+    pp.whole_bin = (int32_t)B.pa.whole_bin;
     pp.use_graph = 0;
     std::memcpy(k.data(), &pp, sizeof(pp));
     return k;
@@ -708,16 +719,19 @@ int search_locked(gx_ctx *c, const gx_dindex *idx, gx_dqueries *q, const gx_sear
                   gx_search_stats *stats, bool want_hist) {
     int st = check_params(p, idx);
     if (st) return st;
-    if (idx->ctx != c || q->ctx != c) return GX_ERR_INVLD_PARAM;
-    if (p->engine != GX_ENGINE_BATCHED && p->engine != GX_ENGINE_SPECTRUM) return GX_ERR_INVLD_PARAM;
+    if (idx->ctx != c || q->ctx != c) return param_error(GX_ERR_INVLD_PARAM, "index or batch of another context");
+    if (p->engine != GX_ENGINE_BATCHED && p->engine != GX_ENGINE_SPECTRUM)
+        return param_error(GX_ERR_INVLD_PARAM, "unknown engine");
     if (hipSetDevice(c->device) != hipSuccess) return GX_ERR_DEVICE;
     // every count bc or yc stays below (2dF+1)*speclen/2 + 1
     if ((int)((2 * p->dF + 1) * (idx->max_speclen / 2) + 1) > c->lgfact_n) return GX_ERR_INVLD_SIZE;
     if (p->evalue_mode == GX_EVALUE_LEARNED) {
         // the surrogate reads the batched engine's per-spectrum histograms
-        if (p->engine != GX_ENGINE_BATCHED) return GX_ERR_INVLD_PARAM;
-        if (!c->surrogate) return GX_ERR_NOT_READY;
-        if (!(p->surrogate_conf >= 0.0f && p->surrogate_conf <= 1.0f)) return GX_ERR_INVLD_PARAM;
+        if (p->engine != GX_ENGINE_BATCHED)
+            return param_error(GX_ERR_INVLD_PARAM, "the learned e-value needs the batched engine");
+        if (!c->surrogate) return param_error(GX_ERR_NOT_READY, "no surrogate attached (gx_ctx_set_surrogate)");
+        if (!(p->surrogate_conf >= 0.0f && p->surrogate_conf <= 1.0f))
+            return param_error(GX_ERR_INVLD_PARAM, "surrogate_conf outside 0..1");
     }
     if (p->engine == GX_ENGINE_SPECTRUM) return search_plain(c, idx, q, p, stats, want_hist);
     return search_batched(c, idx, q, p, stats);
@@ -769,36 +783,56 @@ extern "C" int gx_autotune(gx_ctx *c, const gx_dindex *idx, gx_dqueries *q, cons
     pp.engine = GX_ENGINE_BATCHED;
     pp.use_graph = 0;
     double best = 1e300;
-    int bc = c->tuned_config, bs = c->tuned_split;
-    const int splits[] = {1, 2};
-    for (int cfg = 0; cfg < kNumBatchConfigs; cfg++) {
-        for (int sp : splits) {
-            pp.config = cfg;
-            pp.split = sp;
-            double t = 1e300;
-            for (int rep = 0; rep < 2; rep++) {  // the first run also sizes buffers
-                gx_search_stats st;
-                const int r = search_locked(c, idx, q, &pp, &st, false);
-                if (r) return r;
-                t = std::min(t, st.total_ms);
-            }
-            if (out->n_tried < GX_TUNE_MAX) {
-                out->tried_config[out->n_tried] = cfg;
-                out->tried_split[out->n_tried] = sp;
-                out->tried_ms[out->n_tried] = t;
-                out->n_tried++;
-            }
-            if (t < best) {
-                best = t;
-                bc = cfg;
-                bs = sp;
-            }
+    int bc = c->tuned_config, bs = c->tuned_split, bw = c->tuned_whole;
+    auto trial = [&](int cfg, int sp, int wb, double &t) -> int {
+        pp.config = cfg;
+        pp.split = sp;
+        pp.whole_bin = wb;
+        t = 1e300;
+        for (int rep = 0; rep < 2; rep++) {  // the first run also sizes buffers
+            gx_search_stats st;
+            const int r = search_locked(c, idx, q, &pp, &st, false);
+            if (r) return r;
+            t = std::min(t, st.total_ms);
         }
+        if (out->n_tried < GX_TUNE_MAX) {
+            out->tried_config[out->n_tried] = cfg;
+            out->tried_split[out->n_tried] = sp;
+            out->tried_whole_bin[out->n_tried] = wb;
+            out->tried_ms[out->n_tried] = t;
+            out->n_tried++;
+        }
+        if (t < best) {
+            best = t;
+            bc = cfg;
+            bs = sp;
+            bw = wb;
+        }
+        return GX_SUCCESS;
+    };
+    // launch configuration x window split at the current threshold, then the
+    // threshold for the best pair
+    const int splits[] = {1, 2};
+    const int w0 = bw;
+    for (int cfg = 0; cfg < kNumBatchConfigs; cfg++)
+        for (int sp : splits) {
+            double t;
+            const int r = trial(cfg, sp, w0, t);
+            if (r) return r;
+        }
+    const int cfg1 = bc, sp1 = bs;
+    for (int wb : {64, 128, 192, 256}) {
+        if (wb == w0) continue;
+        double t;
+        const int r = trial(cfg1, sp1, wb, t);
+        if (r) return r;
     }
     c->tuned_config = bc;
     c->tuned_split = bs;
+    c->tuned_whole = bw;
     out->config = bc;
     out->split = bs;
+    out->whole_bin = bw;
     return GX_SUCCESS;
 }
 

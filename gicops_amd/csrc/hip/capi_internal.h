@@ -28,6 +28,7 @@ struct gx_ctx {
     std::vector<hipEvent_t> ev_chunk;  // start/stop per planning chunk
     int tuned_config = 1;
     int tuned_split = 1;
+    int tuned_whole = 128;  // whole-bin threshold (ions)
     uint64_t scratch_gen = 0;  // bumped whenever d_slices / d_cand move (captured graphs hold them)
     double *d_lgfact = nullptr;
     int lgfact_n = 0;
@@ -104,6 +105,8 @@ namespace gx {
 // text of the last HIP failure on this thread (gx_last_error)
 extern thread_local char g_last_error[512];
 void note_error(const char *what, int line, hipError_t e);
+// records "invalid argument: <what>" as the last error text and returns code
+int param_error(int code, const char *what);
 }  // namespace gx
 
 #define GX_HIP(call)                                   \

@@ -37,6 +37,7 @@ class SearchParams:
     config: int = -1
     use_graph: bool = False
     split: int = 0
+    whole_bin: int = 0
 
     def to_ffi(self) -> F.gx_search_params:
         p = F.gx_search_params()
@@ -44,7 +45,7 @@ class SearchParams:
         p.min_shp, p.min_cpsm, p.evalue_mode = self.min_shp, self.min_cpsm, self.evalue_mode
         p.dM, p.expect_max, p.surrogate_conf = self.dM, self.expect_max, self.surrogate_conf
         p.engine, p.config, p.use_graph = self.engine, self.config, int(self.use_graph)
-        p.split = self.split
+        p.split, p.whole_bin = self.split, self.whole_bin
         return p
 
 
@@ -106,13 +107,15 @@ class Searcher:
         return stats_dict(st)
 
     def autotune(self, sample: DeviceQueries, params: SearchParams) -> dict:
-        """Time every launch configuration x window split on `sample` and make
-        the fastest this searcher's default (config=-1, split=0)."""
+        """Time every launch configuration x window split on `sample`, then the
+        whole-bin threshold, and make the fastest this searcher's default
+        (config=-1, split=0, whole_bin=0)."""
         r = F.gx_tune_result()
         p = params.to_ffi()
         F.check("gx_autotune", self._lib.gx_autotune(self._ctx, self._didx, sample._h, C.byref(p), C.byref(r)))
-        return {"config": r.config, "split": r.split,
-                "tried": [(r.tried_config[i], r.tried_split[i], r.tried_ms[i]) for i in range(r.n_tried)]}
+        return {"config": r.config, "split": r.split, "whole_bin": r.whole_bin,
+                "tried": [(r.tried_config[i], r.tried_split[i], r.tried_whole_bin[i], r.tried_ms[i])
+                          for i in range(r.n_tried)]}
 
     def download(self, dq: DeviceQueries, want_hist: bool = False):
         psm = np.zeros(dq.n_spectra, dtype=F.PSM_DTYPE)

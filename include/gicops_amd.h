@@ -112,7 +112,8 @@ typedef struct gx_search_params {
     int32_t config;        /* batched launch configuration, -1 = the context's tuned choice */
     int32_t use_graph;     /* batched: 1 = capture the batch as a hipGraph once, then replay */
     int32_t split;         /* batched: window split, psids per item = tile / split (1, 2 or 4; 0 = tuned) */
-    int32_t reserved;
+    int32_t whole_bin;     /* batched: bins of at most this many ions are streamed whole and filtered by
+                              id instead of searched for tile boundaries (1..4096; 0 = tuned) */
 } gx_search_params;
 
 /* ---- per-spectrum result (hCell + Results + e-value) ----------------- */
@@ -201,22 +202,26 @@ int gx_search(gx_ctx *ctx, const gx_dindex *idx, const gx_query_batch *q, const 
 typedef struct gx_tune_result {
     int32_t config;
     int32_t split;
+    int32_t whole_bin;
     int32_t n_tried;
     int32_t tried_config[GX_TUNE_MAX];
     int32_t tried_split[GX_TUNE_MAX];
+    int32_t tried_whole_bin[GX_TUNE_MAX];
     double tried_ms[GX_TUNE_MAX];
 } gx_tune_result;
 
 /* Time every batched launch configuration x window split on a sample batch
- * (device-resident, as uploaded by gx_queries_upload) and make the fastest
- * the context's default (params.config = -1, params.split = 0). No reference
+ * (device-resident, as uploaded by gx_queries_upload), then the whole-bin
+ * threshold for the fastest pair, and make the winners the context's
+ * defaults (params.config = -1, split = 0, whole_bin = 0). No reference
  * counterpart: the reference's launch shape is fixed (kernel.cu:588-598). */
 int gx_autotune(gx_ctx *ctx, const gx_dindex *idx, gx_dqueries *sample, const gx_search_params *p,
                 gx_tune_result *out);
 
-/* Text of the last HIP failure reported as GX_ERR_DEVICE on this thread:
- * the call, its source line and the HIP error string (the reference prints
- * the CUDA error and exits, cuda/driver.hpp:41-48). */
+/* Text of the last failure on this thread: for GX_ERR_DEVICE the call, its
+ * source line and the HIP error string (the reference prints the CUDA error
+ * and exits, cuda/driver.hpp:41-48); for a rejected search argument, which
+ * one and why. */
 int gx_last_error(char *buf, size_t len);
 
 /* Name of the device the context runs on and its CU count (diagnostics). */

@@ -104,7 +104,9 @@ int main(int argc, char **argv) {
     gx_queries_destroy(gq);
     gx_db_destroy(db);
     if (st) {
-        std::fprintf(stderr, "DSLIM_QuerySpectrum: %d\n", st);
+        char msg[512];
+        gx_last_error(msg, sizeof(msg));
+        std::fprintf(stderr, "DSLIM_QuerySpectrum: %d (%s)\n", st, msg);
         return 7;
     }
     std::printf("adapter_e2e: ok\n");

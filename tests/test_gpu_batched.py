@@ -34,6 +34,18 @@ def test_window_split_parity(open_index, searcher_open, split):
     compare(g, c, hg, hc, f"split {split}")
 
 
+@pytest.mark.parametrize("whole_bin", [1, 256, 4096])
+def test_whole_bin_threshold_parity(open_index, searcher_open, whole_bin):
+    """Bins streamed whole and filtered by id give the same counts as bins
+    searched for their tile boundaries."""
+    say(f"case: whole-bin threshold {whole_bin}, uneven batch")
+    batch = _uneven(open_index, n=150, seed=47)
+    g, hg, st = searcher_open.search(batch, SearchParams(dM=500.0, config=1, whole_bin=whole_bin), want_hist=True)
+    c, hc, cnt = O.search(open_index, batch, O.default_params(dM=500.0), want_hist=True, want_counters=True)
+    check_stats(st, cnt, batch.n_spectra)
+    compare(g, c, hg, hc, f"whole_bin {whole_bin}")
+
+
 def test_graph_replays_match_eager(open_index, searcher_open):
     say("case: hipGraph capture and replays")
     batch = _uneven(open_index, n=150, seed=43)
@@ -63,8 +75,8 @@ def test_autotune_choice_keeps_parity(open_index, searcher_open):
     dq = searcher_open.upload(sample)
     r = searcher_open.autotune(dq, SearchParams(dM=500.0))
     dq.free()
-    assert r["config"] in (0, 1, 2) and r["split"] in (1, 2)
-    assert len(r["tried"]) == 6 and all(ms > 0 for _, _, ms in r["tried"])
+    assert r["config"] in (0, 1, 2) and r["split"] in (1, 2) and r["whole_bin"] in (64, 128, 192, 256)
+    assert len(r["tried"]) == 9 and all(ms > 0 for _, _, _, ms in r["tried"])
     batch = _uneven(open_index, n=100, seed=46)
     g, hg, st = searcher_open.search(batch, SearchParams(dM=500.0), want_hist=True)  # tuned defaults
     assert st["config"] == r["config"]
