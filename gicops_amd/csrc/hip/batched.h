@@ -40,7 +40,6 @@ struct PlanArgs {
     int32_t *pair_hi;
     uint32_t *pair_tiles;    // tiles of the pair (0 = no window)
     uint32_t *pair_pack;     // entries per count word (4 or 2)
-    uint32_t *pair_item_off; // spectrum-local item offset of the pair
     uint32_t *pair_pos_off;  // spectrum-local slice-position offset (words)
     uint32_t *n_items;       // per spectrum, scanned in place to offsets by scan_kernel
     uint64_t *n_pos;         // per spectrum, scanned in place to offsets
@@ -49,7 +48,6 @@ struct PlanArgs {
     uint64_t *pos_off;
     // outputs of plan_slices
     uint32_t *pos;           // chunk-local slice positions: [a, z) per slice
-    uint32_t *items;         // 2 words per item: s, (b | t << 8)
     uint32_t chunk_s0;       // first spectrum of the current chunk
     uint32_t chunk_s1;
     unsigned long long *stats;
@@ -61,6 +59,7 @@ struct ItemArgs {
     const uint32_t *qidx;
     int dF;
     int min_shp;
+    int nbins;
     const double *lgfact;
     int lgfact_n;
     const int32_t *lq;
@@ -73,20 +72,21 @@ struct ItemArgs {
     const uint32_t *pair_pos_off;
     const uint64_t *pos_off;
     const uint32_t *pos;
-    const uint32_t *items;
-    uint32_t item0, item1;   // item range of the chunk
-    uint32_t chunk_s0;
+    uint32_t chunk_s0, chunk_s1;  // spectra of the chunk
+    const uint32_t *order;   // the chunk's spectra in the order they are handed out
     uint32_t *work_counter;  // this chunk's counter
     uint32_t *cand_scratch;  // per block overflow candidates
     int tile_entries_max;
     int split;               // window split: psids per item = tile / split
-    // per-spectrum accumulators
+    uint32_t max_speclen;    // pass-2 task stride, with its exact divisor
+    uint32_t smax_m, smax_s;
+    uint32_t cell_shift;     // bins per coarse peak-index cell = 2^cell_shift
+    // per-spectrum results
     uint32_t *hist;          // n_spectra * GX_HIST_BINS
     uint32_t *cpsms;
-    // per-item top hit: key (h bits << 32 | ~(b << 26 | psid)), equal count, shared ions
-    unsigned long long *item_key;
-    uint32_t *item_eq;
-    uint32_t *item_shp;
+    unsigned long long *best_key;  // top hit: h bits << 32 | ~(b << 26 | psid)
+    uint32_t *best_eq;       // candidates with the top hyperscore
+    uint32_t *best_shp;      // shared ions of the top hit
     unsigned long long *stats;
 };
 
@@ -95,10 +95,9 @@ struct EvalArgs {
     uint32_t n_spectra;
     int min_cpsm;
     double expect_max;
-    const uint32_t *item_off;
-    const unsigned long long *item_key;
-    const uint32_t *item_eq;
-    const uint32_t *item_shp;
+    const unsigned long long *best_key;
+    const uint32_t *best_eq;
+    const uint32_t *best_shp;
     const uint32_t *hist;
     const uint32_t *cpsms;
     const uint8_t *route;    // mode 2: 1 = surrogate result already written
@@ -111,7 +110,7 @@ hipError_t launch_fill_u32(uint32_t *p, uint64_t n, uint32_t v, hipStream_t s);
 hipError_t launch_plan_spectra(const PlanArgs &a, hipStream_t s);
 hipError_t launch_scan(const PlanArgs &a, hipStream_t s);
 hipError_t launch_plan_slices(const PlanArgs &a, hipStream_t s);
-hipError_t launch_search_items(int config, const ItemArgs &a, int n_blocks, hipStream_t s);
+hipError_t launch_search_spectra(int config, const ItemArgs &a, int n_blocks, hipStream_t s);
 hipError_t launch_evalue(const EvalArgs &a, hipStream_t s);
 
 }  // namespace gx

@@ -1,24 +1,24 @@
 // Batched fragment-ion search for gfx950: plan kernels + one persistent,
-// work-stealing search launch over (spectrum, bucket, tile) items + the
-// e-value kernel. Same arithmetic as DSLIM_QuerySpectrum
-// (dslim_query.cpp:749-1081) and ModelTailFit (expeRT.cpp:396-559); the
-// order of work is what changes:
+// work-stealing search launch per planning chunk + the e-value kernel. Same
+// arithmetic as DSLIM_QuerySpectrum (dslim_query.cpp:749-1081) and
+// ModelTailFit (expeRT.cpp:396-559); the order of work is what changes:
 //
-//   plan_spectra  one block per spectrum: legal peaks compacted and sorted by
-//                 m/z, precursor windows of every bucket, the per-spectrum
-//                 bound on shared-peak counts (which picks 8-, 16- or 32-bit
-//                 LDS counters) and the number of psid tiles per window;
-//   scan          item and slice-position offsets per spectrum;
-//   plan_slices   one block per (spectrum, bucket): every (peak, bin) slice
-//                 of iA is cut to the window with two lower_bounds, run by
-//                 thousands of independent blocks so their latency overlaps;
-//   search_items  persistent blocks pull items from a global counter; an item
-//                 streams its slices' ion ids in 64-id pieces (several loads
-//                 in flight per wave), counts shared peaks with LDS atomics,
-//                 rescans candidates against the sorted peaks for exact
-//                 bc/yc/ibc/iyc, and adds its histogram, candidate count and
-//                 best hit to per-spectrum accumulators;
-//   evalue        one wave per spectrum: top hit, tail fit, e-value.
+//   plan_spectra   one block per spectrum: legal peaks compacted and sorted by
+//                  m/z, precursor windows of every bucket, the per-spectrum
+//                  bound on shared-peak counts (which picks 8-, 16- or 32-bit
+//                  LDS counters) and the number of psid tiles per window;
+//   scan           item and slice-position offsets per spectrum;
+//   plan_slices    one block per (bucket, spectrum): every (peak, bin) slice
+//                  of iA is cut to the window with two lower_bounds (small
+//                  bins are flagged to be streamed whole);
+//   search_spectra persistent blocks pull spectra from a counter; each
+//                  (bucket, tile) item of the spectrum streams its slices'
+//                  ion ids in 64-id pieces (several loads in flight per wave)
+//                  and counts shared peaks with LDS atomics; entries with
+//                  enough shared peaks become candidates, rescored once per
+//                  spectrum for exact bc/yc/ibc/iyc; the spectrum's
+//                  histogram, candidate count and top hit are written once;
+//   evalue         one wave per spectrum: top hit, tail fit, e-value.
 #include "batched.h"
 #include "device_util.h"
 
@@ -29,7 +29,8 @@ constexpr int kPlanThreads = 256;
 constexpr int kPeakCap = kMaxPeaksLds;  // 2048 legal peaks per spectrum
 constexpr int kStatSpread = kStatRows;
 constexpr int kUnroll = 4;              // pieces in flight per wave
-constexpr int kItemCandLds = 768;       // candidates kept in LDS per item (the rest in global scratch)
+constexpr int kCandLdsSpec = 640;       // candidates kept in LDS per spectrum (the rest in global scratch)
+constexpr uint32_t kPsidMask = (1u << 26) - 1u;
 constexpr uint32_t kWholeFlag = 0x80000000u;  // marks a whole-bin slice start (ion positions < 2^31)
 
 __device__ __forceinline__ void stat_add(unsigned long long *stats, int which, unsigned long long v) {
@@ -128,7 +129,6 @@ __global__ void __launch_bounds__(kPlanThreads) plan_spectra_kernel(PlanArgs a) 
         const uint64_t nsl = (uint64_t)P * (uint64_t)(2 * dF + 1);
         for (int b = 0; b < a.n_buckets; b++) {
             const uint32_t pi = s * (uint32_t)a.n_buckets + b;
-            a.pair_item_off[pi] = it;
             a.pair_pos_off[pi] = (uint32_t)pw;
             it += s_tiles[b];
             if (s_tiles[b]) pw += nsl * (uint64_t)(s_tiles[b] + 1);
@@ -247,11 +247,6 @@ __global__ void __launch_bounds__(kPlanThreads) plan_slices_kernel(PlanArgs a) {
         }
         bins++;
     }
-    const uint32_t it0 = a.item_off[s] + a.pair_item_off[pi];
-    for (uint32_t t = tid; t < tiles; t += kPlanThreads) {
-        a.items[2 * (uint64_t)(it0 + t)] = s;
-        a.items[2 * (uint64_t)(it0 + t) + 1] = b | (t << 8);
-    }
     bins = wave_sum_u64(bins);
     if (lane == 0) atomicAdd(&s_st[1], bins);
     __syncthreads();
@@ -260,55 +255,27 @@ __global__ void __launch_bounds__(kPlanThreads) plan_slices_kernel(PlanArgs a) {
 
 // ---------------------------------------------------------------------------
 template <int THREADS, int TILE_WORDS, int SL>
-struct ItemLds {
-    alignas(16) uint32_t cnt[TILE_WORDS];  // count tile; pass-2 accumulators after the scan
+struct SpecLds {
+    alignas(16) uint32_t cnt[TILE_WORDS];  // count tile; pass-2 accumulators and the peak index after pass 1
     int32_t pk_q[kPeakCap];
     int32_t pk_i[kPeakCap];
     uint32_t s_a[SL];          // non-empty slices of the tile: first ion
     uint32_t s_flat[SL + 1];   // exclusive prefix of their lengths
+    uint32_t wsum[THREADS / 64];
     uint32_t wsum2[THREADS / 64];
     uint16_t cj0[THREADS];     // slice of each chunk's first id, per round
     unsigned long long cmask[THREADS];  // lanes of each chunk where a later slice starts
     uint32_t ne;
-    uint32_t cand[kItemCandLds];
+    uint32_t cand[kCandLdsSpec];  // the spectrum's candidates, b << 26 | psid (the rest in global scratch)
     uint32_t hist[GX_HIST_BINS];
-    uint32_t wsum[THREADS / 64];
     unsigned long long w_key[THREADS / 64];
     uint32_t w_eq[THREADS / 64];
     uint32_t w_shp[THREADS / 64];
     uint32_t w_cps[THREADS / 64];
-    uint32_t next[2];          // double-buffered prefetched item index
+    uint32_t b_speclen[kMaxBuckets];
+    uint32_t next;             // prefetched next ticket
     uint32_t n_cand;
 };
-
-// exclusive block scan of v[0..n) in place (n <= SL); writes the total to v[n]
-template <int THREADS, int SL>
-__device__ void block_excl_scan(uint32_t *v, int n, uint32_t *wsum) {
-    constexpr int PER = (SL + THREADS - 1) / THREADS;
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    uint32_t loc[PER];
-    uint32_t sum = 0;
-#pragma unroll
-    for (int k = 0; k < PER; k++) {
-        const int i = tid * PER + k;
-        loc[k] = i < n ? v[i] : 0u;
-        sum += loc[k];
-    }
-    const uint32_t inc = wave_incl_scan(sum, lane);
-    if (lane == 63) wsum[wave] = inc;
-    __syncthreads();
-    uint32_t wbase = 0;
-    for (int w = 0; w < wave; w++) wbase += wsum[w];
-    uint32_t run = wbase + inc - sum;
-#pragma unroll
-    for (int k = 0; k < PER; k++) {
-        const int i = tid * PER + k;
-        if (i < n) v[i] = run;
-        run += loc[k];
-    }
-    if (tid == THREADS - 1) v[n] = run;
-    __syncthreads();
-}
 
 // merge two top-hit summaries (key, count of candidates equal to its
 // hyperscore, shared ions of the key's hit)
@@ -328,46 +295,129 @@ __device__ __forceinline__ void merge_best(unsigned long long &k, uint32_t &e, u
     }
 }
 
+// Pass 2 over the spectrum's candidates gathered so far: exact bc/yc/ibc/iyc
+// of each candidate from its own fragment bins (ion_bin) against the sorted
+// peaks, flattened over (candidate, fragment) with a max_speclen stride and
+// summed with LDS atomics; then hyperscore, histogram and top hit. Peaks
+// near a bin are found through a coarse index (first peak of each
+// 2^cell_shift-bin cell) kept in the tail of the count tile.
 template <int THREADS, int TILE_WORDS, int SL>
-__global__ void __launch_bounds__(THREADS) search_items_kernel(ItemArgs a) {
+__device__ void rescore_candidates(const ItemArgs &a, SpecLds<THREADS, TILE_WORDS, SL> &L, int P, uint32_t *g_cand,
+                                   unsigned long long &bkey, uint32_t &beq, uint32_t &bshp, uint32_t &bcps) {
+    constexpr int CB = (TILE_WORDS - kPeakIdxWords) / 3;  // candidates per round (3 accumulators each)
+    const int tid = threadIdx.x;
+    const int dF = a.dF;
+    const uint32_t ncand = L.n_cand;
+    uint16_t *first = reinterpret_cast<uint16_t *>(L.cnt + (TILE_WORDS - kPeakIdxWords));
+    const uint32_t cs = a.cell_shift;
+    const uint32_t ncell = ((uint32_t)a.nbins >> cs) + 1;
+    for (uint32_t c = tid; c < ncell; c += THREADS) first[c] = (uint16_t)lower_bound_lds(L.pk_q, P, (int32_t)(c << cs));
+    uint32_t *acc_n = L.cnt, *acc_ib = L.cnt + CB, *acc_iy = L.cnt + 2 * CB;
+    const uint32_t smax = a.max_speclen;
+    __syncthreads();
+    for (uint32_t c0 = 0; c0 < ncand; c0 += CB) {
+        const uint32_t nc = min((uint32_t)CB, ncand - c0);
+        const uint32_t tasks = nc * smax;
+        constexpr int U = 4;
+        for (uint32_t tb = tid; tb < tasks; tb += U * THREADS) {
+            int32_t x[U];
+            uint32_t cl[U], rr[U], sl[U];
+#pragma unroll
+            for (int u = 0; u < U; u++) {
+                const uint32_t tk = tb + (uint32_t)u * THREADS;
+                const uint32_t c = __umulhi(tk, a.smax_m) >> a.smax_s;
+                const uint32_t r = tk - c * smax;
+                const uint32_t ci = c0 + min(c, nc - 1);
+                const uint32_t cd = ci < (uint32_t)kCandLdsSpec ? L.cand[ci] : gptr(g_cand)[ci - kCandLdsSpec];
+                const uint32_t b = cd >> 26, psid = cd & kPsidMask;
+                const uint32_t spl = L.b_speclen[b];
+                const bool ok = tk < tasks && r < spl;
+                cl[u] = ok ? c : 0xffffffffu;
+                rr[u] = r;
+                sl[u] = spl;
+                x[u] = ok ? (int32_t)gptr(a.buckets[b].ion_bin)[(size_t)psid * spl + r] : 0;
+            }
+#pragma unroll
+            for (int u = 0; u < U; u++) {
+                if (cl[u] == 0xffffffffu) continue;
+                const int32_t lo = x[u] - dF, hi = x[u] + dF;
+                int k = first[(uint32_t)max(lo, 0) >> cs];
+                while (k < P && L.pk_q[k] < lo) k++;
+                uint32_t n = 0, sum = 0;
+                for (; k < P && L.pk_q[k] <= hi; k++) {
+                    n++;
+                    sum += (uint32_t)L.pk_i[k];
+                }
+                if (n) {
+                    const bool isy = rr[u] >= (sl[u] >> 1);
+                    atomicAdd(&acc_n[cl[u]], isy ? (n << 16) : n);
+                    atomicAdd(isy ? &acc_iy[cl[u]] : &acc_ib[cl[u]], sum);
+                }
+            }
+        }
+        __syncthreads();
+        for (uint32_t c = tid; c < nc; c += THREADS) {
+            const uint32_t v = acc_n[c];
+            const uint32_t bc = v & 0xffffu, yc = v >> 16;
+            const uint32_t ibc = acc_ib[c], iyc = acc_iy[c];
+            acc_n[c] = 0;
+            acc_ib[c] = 0;
+            acc_iy[c] = 0;
+            const uint32_t ci = c0 + c;
+            const uint32_t cd = ci < (uint32_t)kCandLdsSpec ? L.cand[ci] : gptr(g_cand)[ci - kCandLdsSpec];
+            const uint32_t shpk = bc + yc;
+            if (shpk >= (uint32_t)a.min_shp && (int)bc < a.lgfact_n && (int)yc < a.lgfact_n) {
+                float hs = hyperscore(bc, yc, ibc, iyc, a.lgfact);
+                if (hs > 0) {
+                    if (hs >= 80.0f) hs = 80.0f - 1;
+                    bcps++;
+                    atomicAdd(&L.hist[hyp_bin(hs)], 1u);
+                    merge_best(bkey, beq, bshp, hit_key(hs, cd >> 26, cd & kPsidMask), 1u, shpk);
+                }
+            }
+        }
+        __syncthreads();
+    }
+    for (uint32_t c = tid; c < (ncell + 1) / 2; c += THREADS) L.cnt[TILE_WORDS - kPeakIdxWords + c] = 0;
+    if (tid == 0) L.n_cand = 0;
+    __syncthreads();
+}
+
+// Persistent search: blocks pull whole spectra of the chunk from a counter.
+// For each of the spectrum's (bucket, tile) items the block streams the
+// tile's slices of ion ids (pass 1: shared-peak counts in the packed LDS
+// tile), then collects the tile's entries with count >= min_shp as
+// candidates. Pass 2 rescans all candidates of the spectrum at once; the
+// histogram, candidate count and top hit are written once per spectrum.
+template <int THREADS, int TILE_WORDS, int SL, int WAVES_PER_SIMD>
+__global__ void __launch_bounds__(THREADS, WAVES_PER_SIMD) search_spectra_kernel(ItemArgs a) {
     constexpr int NW = THREADS / 64;
-    constexpr int CB = TILE_WORDS / 3;  // candidates per pass-2 round (3 LDS accumulators each)
-    __shared__ ItemLds<THREADS, TILE_WORDS, SL> L;
+    __shared__ SpecLds<THREADS, TILE_WORDS, SL> L;
     const int tid = threadIdx.x, lane = tid & 63;
     const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
     const int dF = a.dF, nb5 = 2 * dF + 1;
     const uint32_t min_cnt = (uint32_t)max(a.min_shp, 1);
+    const uint32_t nbk = (uint32_t)a.n_buckets;
     uint32_t *g_cand = a.cand_scratch + (size_t)blockIdx.x * a.tile_entries_max;
+    const uint32_t cand_cap = (uint32_t)kCandLdsSpec + (uint32_t)a.tile_entries_max;
     for (int w = tid; w < TILE_WORDS; w += THREADS) L.cnt[w] = 0;
     for (int w = tid; w < GX_HIST_BINS; w += THREADS) L.hist[w] = 0;
-    if (tid == 0) L.next[0] = a.item0 + atomicAdd(a.work_counter, 1u);
+    for (int b = tid; b < a.n_buckets; b += THREADS) L.b_speclen[b] = a.buckets[b].speclen;
+    const uint32_t nspec = a.chunk_s1 - a.chunk_s0;
+    if (tid == 0) {
+        L.next = atomicAdd(a.work_counter, 1u);
+        L.n_cand = 0;
+    }
     unsigned long long my_cand = 0, my_scanned = 0, my_errors = 0;
 
-    for (uint32_t par = 0;; par ^= 1u) {
+    for (;;) {
         __syncthreads();
-        const uint32_t item = L.next[par];
-        if (item >= a.item1) break;
-        // fetch the next item now; its latency overlaps this one
-        if (tid == 0) L.next[par ^ 1u] = a.item0 + atomicAdd(a.work_counter, 1u);
-        const uint32_t s = a.items[2 * (uint64_t)item];
-        const uint32_t bt = a.items[2 * (uint64_t)item + 1];
-        const uint32_t b = bt & 0xffu, t = bt >> 8;
-        const uint32_t pi = s * (uint32_t)a.n_buckets + b;
-        const int32_t lo = a.pair_lo[pi], hi = a.pair_hi[pi];
-        const uint32_t tiles = a.pair_tiles[pi];
-        const uint32_t pack = a.pair_pack[pi];
-        const uint32_t lp = pack == 4 ? 2u : (pack == 2 ? 1u : 0u);  // log2 entries per word
-        const uint32_t fbits = 32u >> lp;                             // counter width
-        const uint32_t fmask = fbits == 32 ? 0xffffffffu : ((1u << fbits) - 1u);
-        const uint32_t T = ((uint32_t)TILE_WORDS / (uint32_t)a.split) << lp;
-        const int32_t tlo = lo + (int32_t)(t * T);
-        const int32_t thi = min(hi, tlo + (int32_t)T - 1);
-        const uint32_t nent = (uint32_t)(thi - tlo + 1);
-        const DevBucket bk = a.buckets[b];
-        const uint32_t speclen = bk.speclen;
-        const uint32_t tlo_id = (uint32_t)tlo * speclen;
-        const uint32_t span_id = nent * speclen;
-        const uint32_t fsh = 5u - lp;  // log2 of the counter width
+        const uint32_t ticket = L.next;
+        if (ticket >= nspec) break;
+        const uint32_t s = a.order[a.chunk_s0 + ticket];
+        __syncthreads();
+        // fetch the next ticket now; its latency overlaps this spectrum
+        if (tid == 0) L.next = atomicAdd(a.work_counter, 1u);
         const int P = (int)a.n_legal[s];
         const uint32_t qoff = a.qidx[s];
         for (int k = tid; k < P; k += THREADS) {
@@ -375,211 +425,189 @@ __global__ void __launch_bounds__(THREADS) search_items_kernel(ItemArgs a) {
             L.pk_i[k] = a.li[qoff + k];
         }
         const int nsl = P * nb5;
-        const uint32_t stride = tiles + 1;
-        const uint32_t *pos = a.pos + (a.pos_off[s] - a.pos_off[a.chunk_s0]) + a.pair_pos_off[pi];
-        const auto ia = gptr(bk.iA);
-        // ---- pass 1: shared-peak counts of every entry in the tile ----
-        const uint32_t dm = bk.div_m, dsh = bk.div_s;
+        unsigned long long bkey = 0;
+        uint32_t beq = 0, bshp = 0, bcps = 0;
         uint32_t n_scanned = 0, n_errors = 0;
-        for (int c0 = 0; c0 < nsl; c0 += SL) {
-            const int ns = min(SL, nsl - c0);
-            // slice ranges of this tile; non-empty ones compacted in order
-            static_assert(SL == THREADS, "one slice per thread");
-            uint32_t at = 0, len = 0;
-            if (tid < ns) {
-                const uint32_t *ps = pos + (uint64_t)(c0 + tid) * stride;
-                const uint32_t p0 = ps[0];
-                if (p0 & kWholeFlag) {  // small bin: the whole bin, filtered by id below
-                    at = p0 & ~kWholeFlag;
-                    len = ps[1] - at;
-                } else {
-                    at = ps[t];
-                    len = ps[t + 1] - at;
-                }
-            }
-            const uint32_t fl = len ? 1u : 0u;
-            const uint32_t il = wave_incl_scan(len, lane), ifl = wave_incl_scan(fl, lane);
-            if (lane == 63) {
-                L.wsum[wave] = il;
-                L.wsum2[wave] = ifl;
-            }
-            __syncthreads();
-            uint32_t bl = 0, bf = 0;
-            for (int w = 0; w < wave; w++) {
-                bl += L.wsum[w];
-                bf += L.wsum2[w];
-            }
-            if (fl) {
-                L.s_a[bf + ifl - 1] = at;
-                L.s_flat[bf + ifl - 1] = bl + il - len;
-            }
-            if (tid == THREADS - 1) {
-                L.s_flat[bf + ifl] = bl + il;
-                L.ne = bf + ifl;
-            }
-            __syncthreads();
-            // the tile's ions, flattened over the non-empty slices, in rounds
-            // of THREADS 64-id chunks. One thread per chunk finds the slice
-            // holding the chunk's first id and a bit mask of the lanes where
-            // later slices start.
-            const uint32_t ne = L.ne;
-            const uint32_t N = L.s_flat[ne];
-            const uint32_t nch = (N + 63) >> 6;
-            for (uint32_t r0 = 0; r0 < nch; r0 += THREADS) {
-                const uint32_t nr = min((uint32_t)THREADS, nch - r0);
-                if ((uint32_t)tid < nr) {
-                    const uint32_t f0 = (r0 + tid) << 6;
-                    const uint32_t j = (uint32_t)upper_bound_lds(L.s_flat, (int)ne + 1, f0) - 1;
-                    unsigned long long mk = 0;
-                    for (uint32_t k = j + 1; k < ne; k++) {
-                        const uint32_t d = L.s_flat[k] - f0;
-                        if (d >= 64) break;
-                        mk |= 1ull << d;
+        const uint64_t pos_s = a.pos_off[s] - a.pos_off[a.chunk_s0];
+        for (uint32_t b = 0; b < nbk; b++) {
+            const uint32_t pi = s * nbk + b;
+            const uint32_t tiles = a.pair_tiles[pi];
+            if (tiles == 0) continue;
+            const int32_t lo = a.pair_lo[pi], hi = a.pair_hi[pi];
+            const uint32_t pack = a.pair_pack[pi];
+            const uint32_t lp = pack == 4 ? 2u : (pack == 2 ? 1u : 0u);  // log2 entries per word
+            const uint32_t fbits = 32u >> lp;                             // counter width
+            const uint32_t fmask = fbits == 32 ? 0xffffffffu : ((1u << fbits) - 1u);
+            const uint32_t fsh = 5u - lp;                                 // log2 of the counter width
+            const uint32_t T = ((uint32_t)TILE_WORDS / (uint32_t)a.split) << lp;
+            const DevBucket bk = a.buckets[b];
+            const uint32_t speclen = bk.speclen;
+            const uint32_t stride = tiles + 1;
+            const uint32_t *pos = a.pos + pos_s + a.pair_pos_off[pi];
+            const auto ia = gptr(bk.iA);
+            const uint32_t dm = bk.div_m, dsh = bk.div_s;
+            for (uint32_t t = 0; t < tiles; t++) {
+                const int32_t tlo = lo + (int32_t)(t * T);
+                const int32_t thi = min(hi, tlo + (int32_t)T - 1);
+                const uint32_t nent = (uint32_t)(thi - tlo + 1);
+                const uint32_t tlo_id = (uint32_t)tlo * speclen;
+                const uint32_t span_id = nent * speclen;
+                // room for every entry of this tile as a candidate
+                __syncthreads();
+                if (L.n_cand + nent > cand_cap)
+                    rescore_candidates<THREADS, TILE_WORDS, SL>(a, L, P, g_cand, bkey, beq, bshp, bcps);
+                // ---- pass 1: shared-peak counts of every entry in the tile ----
+                for (int c0 = 0; c0 < nsl; c0 += SL) {
+                    const int ns = min(SL, nsl - c0);
+                    // slice ranges of this tile; non-empty ones compacted in order
+                    static_assert(SL == THREADS, "one slice per thread");
+                    uint32_t at = 0, len = 0;
+                    if (tid < ns) {
+                        const uint32_t *ps = pos + (uint64_t)(c0 + tid) * stride;
+                        const uint32_t p0 = ps[0];
+                        if (p0 & kWholeFlag) {  // small bin: the whole bin, filtered by id below
+                            at = p0 & ~kWholeFlag;
+                            len = ps[1] - at;
+                        } else {
+                            at = ps[t];
+                            len = ps[t + 1] - at;
+                        }
                     }
-                    L.cj0[tid] = (uint16_t)j;
-                    L.cmask[tid] = mk;
+                    const uint32_t fl = len ? 1u : 0u;
+                    const uint32_t il = wave_incl_scan(len, lane), ifl = wave_incl_scan(fl, lane);
+                    if (lane == 63) {
+                        L.wsum[wave] = il;
+                        L.wsum2[wave] = ifl;
+                    }
+                    __syncthreads();
+                    uint32_t bl = 0, bf = 0;
+                    for (int w = 0; w < wave; w++) {
+                        bl += L.wsum[w];
+                        bf += L.wsum2[w];
+                    }
+                    if (fl) {
+                        L.s_a[bf + ifl - 1] = at;
+                        L.s_flat[bf + ifl - 1] = bl + il - len;
+                    }
+                    if (tid == THREADS - 1) {
+                        L.s_flat[bf + ifl] = bl + il;
+                        L.ne = bf + ifl;
+                    }
+                    __syncthreads();
+                    // the tile's ions, flattened over the non-empty slices, in rounds
+                    // of THREADS 64-id chunks. One thread per chunk finds the slice
+                    // holding the chunk's first id and a bit mask of the lanes where
+                    // later slices start.
+                    const uint32_t ne = L.ne;
+                    const uint32_t N = L.s_flat[ne];
+                    const uint32_t nch = (N + 63) >> 6;
+                    for (uint32_t r0 = 0; r0 < nch; r0 += THREADS) {
+                        const uint32_t nr = min((uint32_t)THREADS, nch - r0);
+                        if ((uint32_t)tid < nr) {
+                            const uint32_t f0 = (r0 + tid) << 6;
+                            const uint32_t j = (uint32_t)upper_bound_lds(L.s_flat, (int)ne + 1, f0) - 1;
+                            unsigned long long mk = 0;
+                            for (uint32_t k = j + 1; k < ne; k++) {
+                                const uint32_t d = L.s_flat[k] - f0;
+                                if (d >= 64) break;
+                                mk |= 1ull << d;
+                            }
+                            L.cj0[tid] = (uint16_t)j;
+                            L.cmask[tid] = mk;
+                        }
+                        __syncthreads();
+                        // U chunks per wave in three batched LDS/HBM stages: chunk
+                        // metadata, slice bases, ion ids; lane L's slice is the chunk's
+                        // first plus the slice starts at lanes <= L
+                        constexpr int U = 2 * kUnroll;
+                        const unsigned long long upto =
+                            lane == 63 ? ~0ull : ((1ull << (uint32_t)(lane + 1)) - 1ull);
+                        for (uint32_t c = (uint32_t)wave * U; c < nr; c += NW * U) {
+                            uint32_t j0[U], sa[U], sf[U], raw[U];
+                            unsigned long long mk[U];
+#pragma unroll
+                            for (int u = 0; u < U; u++) {
+                                const uint32_t cc = min(c + (uint32_t)u, nr - 1);
+                                j0[u] = L.cj0[cc];
+                                mk[u] = L.cmask[cc];
+                            }
+#pragma unroll
+                            for (int u = 0; u < U; u++) {
+                                const uint32_t j = j0[u] + (uint32_t)__popcll(mk[u] & upto);
+                                sa[u] = L.s_a[j];
+                                sf[u] = L.s_flat[j];
+                            }
+#pragma unroll
+                            for (int u = 0; u < U; u++) {
+                                const uint32_t f = ((r0 + c + (uint32_t)u) << 6) + (uint32_t)lane;
+                                const bool ok = c + (uint32_t)u < nr && f < N;
+                                // a failed lane gets an id just below the tile: filtered below
+                                raw[u] = ok ? ia[sa[u] + (f - sf[u])] : tlo_id - 1u;
+                            }
+#pragma unroll
+                            for (int u = 0; u < U; u++) {
+                                // ids of the tile's entries: [tlo_id, tlo_id + span_id), so
+                                // el < nent follows; the guard only keeps LDS writes in the tile
+                                if (raw[u] - tlo_id < span_id) {
+                                    const uint32_t el = (__umulhi(raw[u], dm) >> dsh) - (uint32_t)tlo;
+                                    if (el < nent) {
+                                        atomicAdd(&L.cnt[el >> lp], 1u << ((el & (pack - 1)) << fsh));
+                                        n_scanned++;
+                                    } else {
+                                        n_errors++;  // never expected (exact division is checked on the host)
+                                    }
+                                }
+                            }
+                        }
+                        __syncthreads();
+                    }
                 }
                 __syncthreads();
-                // U chunks per wave in three batched LDS/HBM stages: chunk
-                // metadata, slice bases, ion ids; lane L's slice is the chunk's
-                // first plus the slice starts at lanes <= L
-                constexpr int U = 2 * kUnroll;
-                const unsigned long long upto = lane == 63 ? ~0ull : ((1ull << (uint32_t)(lane + 1)) - 1ull);
-                for (uint32_t c = (uint32_t)wave * U; c < nr; c += NW * U) {
-                    uint32_t j0[U], sa[U], sf[U], raw[U];
-                    unsigned long long mk[U];
+                // ---- candidates: entries with count >= min_shp; clear the tile ----
+                {
+                    // SWAR "some counter >= min_cnt" test per word (min_cnt <= 128)
+                    const uint32_t ones = pack == 4 ? 0x01010101u : (pack == 2 ? 0x00010001u : 1u);
+                    const uint32_t hib = ones << (fbits - 1);
+                    const uint32_t low = pack == 1 ? 0x7fffffffu : (~hib);
+                    const bool swar = min_cnt <= 128;
+                    const uint32_t addv = ((1u << (fbits - 1)) - min_cnt) * ones;
+                    const uint32_t nw = (nent + pack - 1) >> lp;
+                    const uint32_t nq = (nw + 3) >> 2;
+                    const uint32_t btag = b << 26;
+                    uint4 *cq = reinterpret_cast<uint4 *>(L.cnt);
+                    for (uint32_t q = tid; q < nq; q += THREADS) {
+                        const uint4 v4 = cq[q];
+                        if ((v4.x | v4.y | v4.z | v4.w) == 0) continue;
+                        cq[q] = make_uint4(0, 0, 0, 0);
+                        const uint32_t vv[4] = {v4.x, v4.y, v4.z, v4.w};
 #pragma unroll
-                    for (int u = 0; u < U; u++) {
-                        const uint32_t cc = min(c + (uint32_t)u, nr - 1);
-                        j0[u] = L.cj0[cc];
-                        mk[u] = L.cmask[cc];
-                    }
-#pragma unroll
-                    for (int u = 0; u < U; u++) {
-                        const uint32_t j = j0[u] + (uint32_t)__popcll(mk[u] & upto);
-                        sa[u] = L.s_a[j];
-                        sf[u] = L.s_flat[j];
-                    }
-#pragma unroll
-                    for (int u = 0; u < U; u++) {
-                        const uint32_t f = ((r0 + c + (uint32_t)u) << 6) + (uint32_t)lane;
-                        const bool ok = c + (uint32_t)u < nr && f < N;
-                        // a failed lane gets an id just below the tile: filtered below
-                        raw[u] = ok ? ia[sa[u] + (f - sf[u])] : tlo_id - 1u;
-                    }
-#pragma unroll
-                    for (int u = 0; u < U; u++) {
-                        // ids of the tile's entries: [tlo_id, tlo_id + span_id), so
-                        // el < nent follows; the guard only keeps LDS writes in the tile
-                        if (raw[u] - tlo_id < span_id) {
-                            const uint32_t el = (__umulhi(raw[u], dm) >> dsh) - (uint32_t)tlo;
-                            if (el < nent) {
-                                atomicAdd(&L.cnt[el >> lp], 1u << ((el & (pack - 1)) << fsh));
-                                n_scanned++;
-                            } else {
-                                n_errors++;  // never expected (exact division is checked on the host)
+                        for (int i = 0; i < 4; i++) {
+                            const uint32_t v = vv[i];
+                            if (v == 0) continue;
+                            if (swar && ((((v & low) + addv) | v) & hib) == 0) continue;
+                            const uint32_t w = 4 * q + i;
+                            for (uint32_t e = 0; e < pack; e++) {
+                                const uint32_t cc = (v >> (e * fbits)) & fmask;
+                                const uint32_t el = (w << lp) + e;
+                                if (cc >= min_cnt && el < nent) {
+                                    const uint32_t p = atomicAdd(&L.n_cand, 1u);
+                                    const uint32_t cd = btag | ((uint32_t)tlo + el);
+                                    if (p < (uint32_t)kCandLdsSpec)
+                                        L.cand[p] = cd;
+                                    else
+                                        g_cand[p - kCandLdsSpec] = cd;
+                                }
                             }
                         }
                     }
                 }
-                __syncthreads();
             }
-            __syncthreads();
         }
+        __syncthreads();
+        if (L.n_cand) rescore_candidates<THREADS, TILE_WORDS, SL>(a, L, P, g_cand, bkey, beq, bshp, bcps);
         my_scanned += n_scanned;
         my_errors += n_errors;
-        // ---- candidates: entries with count >= min_shp; clear the tile ----
-        if (tid == 0) L.n_cand = 0;
-        __syncthreads();
-        {
-            // SWAR "some counter >= min_cnt" test per word (min_cnt <= 128)
-            const uint32_t ones = pack == 4 ? 0x01010101u : (pack == 2 ? 0x00010001u : 1u);
-            const uint32_t hib = ones << (fbits - 1);
-            const uint32_t low = pack == 1 ? 0x7fffffffu : (~hib);
-            const bool swar = min_cnt <= 128;
-            const uint32_t addv = ((1u << (fbits - 1)) - min_cnt) * ones;
-            const uint32_t nw = (nent + pack - 1) >> lp;
-            const uint32_t nq = (nw + 3) >> 2;
-            uint4 *cq = reinterpret_cast<uint4 *>(L.cnt);
-            for (uint32_t q = tid; q < nq; q += THREADS) {
-                const uint4 v4 = cq[q];
-                if ((v4.x | v4.y | v4.z | v4.w) == 0) continue;
-                cq[q] = make_uint4(0, 0, 0, 0);
-                const uint32_t vv[4] = {v4.x, v4.y, v4.z, v4.w};
-#pragma unroll
-                for (int i = 0; i < 4; i++) {
-                    const uint32_t v = vv[i];
-                    if (v == 0) continue;
-                    if (swar && ((((v & low) + addv) | v) & hib) == 0) continue;
-                    const uint32_t w = 4 * q + i;
-                    for (uint32_t e = 0; e < pack; e++) {
-                        const uint32_t cc = (v >> (e * fbits)) & fmask;
-                        const uint32_t el = (w << lp) + e;
-                        if (cc >= min_cnt && el < nent) {
-                            const uint32_t p = atomicAdd(&L.n_cand, 1u);
-                            const uint32_t psid = (uint32_t)tlo + el;
-                            if (p < (uint32_t)kItemCandLds)
-                                L.cand[p] = psid;
-                            else
-                                g_cand[p - kItemCandLds] = psid;
-                        }
-                    }
-                }
-            }
-        }
-        __syncthreads();
-        // ---- pass 2: exact bc/yc/ibc/iyc of every candidate, flattened over
-        // (candidate, fragment ion) and summed with LDS atomics ----
-        const uint32_t ncand = L.n_cand;
-        const uint32_t half = speclen >> 1;
-        unsigned long long bkey = 0;
-        uint32_t beq = 0, bshp = 0, bcps = 0;
-        uint32_t *acc_n = L.cnt, *acc_ib = L.cnt + CB, *acc_iy = L.cnt + 2 * CB;
-        for (uint32_t c0 = 0; c0 < ncand; c0 += CB) {
-            const uint32_t nc = min((uint32_t)CB, ncand - c0);
-            const uint32_t tasks = nc * speclen;
-            for (uint32_t tk = tid; tk < tasks; tk += THREADS) {
-                const uint32_t c = __umulhi(tk, bk.div_m) >> bk.div_s;
-                const uint32_t r = tk - c * speclen;
-                const uint32_t ci = c0 + c;
-                const uint32_t psid = ci < (uint32_t)kItemCandLds ? L.cand[ci] : gptr(g_cand)[ci - kItemCandLds];
-                const int32_t x = (int32_t)gptr(bk.ion_bin)[(size_t)psid * speclen + r];
-                int k = lower_bound_lds(L.pk_q, P, x - dF);
-                uint32_t n = 0, sum = 0;
-                for (; k < P && L.pk_q[k] <= x + dF; k++) {
-                    n++;
-                    sum += (uint32_t)L.pk_i[k];
-                }
-                if (n) {
-                    const bool isy = r >= half;
-                    atomicAdd(&acc_n[c], isy ? (n << 16) : n);
-                    atomicAdd(isy ? &acc_iy[c] : &acc_ib[c], sum);
-                }
-            }
-            __syncthreads();
-            for (uint32_t c = tid; c < nc; c += THREADS) {
-                const uint32_t v = acc_n[c];
-                const uint32_t bc = v & 0xffffu, yc = v >> 16;
-                const uint32_t ibc = acc_ib[c], iyc = acc_iy[c];
-                acc_n[c] = 0;
-                acc_ib[c] = 0;
-                acc_iy[c] = 0;
-                const uint32_t ci = c0 + c;
-                const uint32_t psid = ci < (uint32_t)kItemCandLds ? L.cand[ci] : gptr(g_cand)[ci - kItemCandLds];
-                const uint32_t shpk = bc + yc;
-                if (shpk >= (uint32_t)a.min_shp && (int)bc < a.lgfact_n && (int)yc < a.lgfact_n) {
-                    float hs = hyperscore(bc, yc, ibc, iyc, a.lgfact);
-                    if (hs > 0) {
-                        if (hs >= 80.0f) hs = 80.0f - 1;
-                        bcps++;
-                        atomicAdd(&L.hist[hyp_bin(hs)], 1u);
-                        merge_best(bkey, beq, bshp, hit_key(hs, b, psid), 1u, shpk);
-                    }
-                }
-            }
-            __syncthreads();
-        }
-        // ---- block reduction of the top hit, flush to the spectrum ----
+        // ---- block reduction of the top hit; the spectrum's results ----
 #pragma unroll
         for (int o = 32; o > 0; o >>= 1) {
             const unsigned long long k2 = __shfl_xor(bkey, o, 64);
@@ -595,11 +623,8 @@ __global__ void __launch_bounds__(THREADS) search_items_kernel(ItemArgs a) {
         }
         __syncthreads();
         for (int w = tid; w < GX_HIST_BINS; w += THREADS) {
-            const uint32_t v = L.hist[w];
-            if (v) {
-                atomicAdd(&a.hist[(size_t)s * GX_HIST_BINS + w], v);
-                L.hist[w] = 0;
-            }
+            a.hist[(size_t)s * GX_HIST_BINS + w] = L.hist[w];
+            L.hist[w] = 0;
         }
         if (tid == 0) {
             unsigned long long key = 0;
@@ -608,10 +633,10 @@ __global__ void __launch_bounds__(THREADS) search_items_kernel(ItemArgs a) {
                 cps += L.w_cps[w];
                 merge_best(key, eq, shp, L.w_key[w], L.w_eq[w], L.w_shp[w]);
             }
-            a.item_key[item] = key;
-            a.item_eq[item] = eq;
-            a.item_shp[item] = shp;
-            if (cps) atomicAdd(&a.cpsms[s], cps);
+            a.best_key[s] = key;
+            a.best_eq[s] = eq;
+            a.best_shp[s] = shp;
+            a.cpsms[s] = cps;
             my_cand += cps;
         }
     }
@@ -641,21 +666,8 @@ __global__ void __launch_bounds__(64) evalue_kernel(EvalArgs a) {
         r = a.out[s];
         if (a.route[s] || !(r.flags & GX_PSM_MODELED)) return;
     } else {
-        const uint32_t i0 = a.item_off[s], i1 = a.item_off[s + 1];
-        unsigned long long key = 0;
-        for (uint32_t i = i0 + lane; i < i1; i += 64) {
-            const unsigned long long k = a.item_key[i];
-            key = k > key ? k : key;
-        }
-        key = wave_max_u64(key);
-        uint32_t eq = 0, shp = 0;
-        for (uint32_t i = i0 + lane; i < i1; i += 64) {
-            const unsigned long long k = a.item_key[i];
-            if (k && (k >> 32) == (key >> 32)) eq += a.item_eq[i];
-            if (k && k == key) shp = a.item_shp[i];
-        }
-        eq = wave_sum_u32(eq);
-        shp = (uint32_t)wave_max_i32((int)shp);
+        const unsigned long long key = a.best_key[s];
+        const uint32_t eq = a.best_eq[s], shp = a.best_shp[s];
         const uint32_t cps = a.cpsms[s];
         r.hyperscore = 0.0f;
         r.psid = -1;
@@ -700,9 +712,12 @@ __global__ void __launch_bounds__(64) evalue_kernel(EvalArgs a) {
     if (lane == 0) a.out[s] = r;
 }
 
-template <int THREADS, int TILE_WORDS, int SL>
-hipError_t launch_items(const ItemArgs &a, int n_blocks, hipStream_t s) {
-    hipLaunchKernelGGL((search_items_kernel<THREADS, TILE_WORDS, SL>), dim3(n_blocks), dim3(THREADS), 0, s, a);
+// WAVES_PER_SIMD = waves of the resident blocks per SIMD (4 SIMDs per CU):
+// it caps the VGPRs so that blocks_per_cu blocks fit
+template <int THREADS, int TILE_WORDS, int SL, int WAVES_PER_SIMD>
+hipError_t launch_spectra(const ItemArgs &a, int n_blocks, hipStream_t s) {
+    hipLaunchKernelGGL((search_spectra_kernel<THREADS, TILE_WORDS, SL, WAVES_PER_SIMD>), dim3(n_blocks), dim3(THREADS),
+                       0, s, a);
     return hipGetLastError();
 }
 
@@ -743,11 +758,11 @@ hipError_t launch_plan_slices(const PlanArgs &a, hipStream_t s) {
     return hipGetLastError();
 }
 
-hipError_t launch_search_items(int config, const ItemArgs &a, int n_blocks, hipStream_t s) {
+hipError_t launch_search_spectra(int config, const ItemArgs &a, int n_blocks, hipStream_t s) {
     switch (config) {
-        case 0: return launch_items<1024, 28672, 1024>(a, n_blocks, s);
-        case 1: return launch_items<512, 12288, 512>(a, n_blocks, s);
-        default: return launch_items<256, 4096, 256>(a, n_blocks, s);
+        case 0: return launch_spectra<1024, 28672, 1024, 4>(a, n_blocks, s);
+        case 1: return launch_spectra<512, 12288, 512, 4>(a, n_blocks, s);
+        default: return launch_spectra<256, 4096, 256, 3>(a, n_blocks, s);
     }
 }
 

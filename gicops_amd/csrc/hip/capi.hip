@@ -217,6 +217,8 @@ extern "C" int gx_index_upload(gx_ctx *c, const gx_bucket *b, int nb, int maxz, 
         }
         d->h_buckets[i] = db;
     }
+    if (!make_divisor(std::max(d->max_speclen, 2u), d->smax_m, d->smax_s)) return fail(GX_ERR_INVLD_INDEX);
+    while ((nbins >> d->cell_shift) + 1 > 2u * (uint32_t)kPeakIdxWords) d->cell_shift++;
     if (dalloc(&d->d_buckets, nb)) return fail(GX_ERR_BAD_MEM_ALLOC);
     if (hipMemcpy(d->d_buckets, d->h_buckets.data(), sizeof(DevBucket) * nb, hipMemcpyHostToDevice) != hipSuccess)
         return fail(GX_ERR_DEVICE);
@@ -287,9 +289,9 @@ extern "C" void gx_queries_free(gx_dqueries *d) {
     if (d->graph) (void)hipGraphDestroy(d->graph);
     void *ps[] = {d->idx,       d->moz,        d->inten,         d->prec,         d->dm,       d->psm,
                   d->hist,      d->lq,         d->li,            d->n_legal,      d->pair_lo,  d->pair_hi,
-                  d->pair_tiles, d->pair_pack, d->pair_item_off, d->pair_pos_off, d->n_items,  d->item_off,
-                  d->n_pos,     d->pos_off,    d->cpsms,         d->counters,     d->items,    d->item_key,
-                  d->item_eq,   d->item_shp,   d->pos,           d->l_feat,       d->l_h1,     d->l_h2,
+                  d->pair_tiles, d->pair_pack, d->pair_pos_off, d->n_items,     d->item_off, d->n_pos,
+                  d->pos_off,   d->cpsms,      d->counters,      d->best_key,     d->best_eq,  d->best_shp,
+                  d->pos,       d->order,           d->l_feat,       d->l_h1,     d->l_h2,
                   d->l_anchor,  d->l_tgt,      d->l_loge,     d->l_out,         d->l_valid,      d->l_sparse, d->l_route};
     for (void *p : ps)
         if (p) (void)hipFree(p);
@@ -389,7 +391,8 @@ int ensure_batched(gx_dqueries *q, const gx_dindex *idx) {
     q->nb = idx->n_buckets;
     if (dalloc(&q->lq, np) || dalloc(&q->li, np) || dalloc(&q->n_legal, ns) || dalloc(&q->pair_lo, npairs) ||
         dalloc(&q->pair_hi, npairs) || dalloc(&q->pair_tiles, npairs) || dalloc(&q->pair_pack, npairs) ||
-        dalloc(&q->pair_item_off, npairs) || dalloc(&q->pair_pos_off, npairs) || dalloc(&q->n_items, ns) ||
+        dalloc(&q->pair_pos_off, npairs) || dalloc(&q->n_items, ns) || dalloc(&q->best_key, ns) || dalloc(&q->order, ns) ||
+        dalloc(&q->best_eq, ns) || dalloc(&q->best_shp, ns) ||
         dalloc(&q->item_off, ns + 1) || dalloc(&q->n_pos, ns) || dalloc(&q->pos_off, ns + 1) ||
         dalloc(&q->cpsms, ns) || (!q->hist && dalloc(&q->hist, ns * GX_HIST_BINS)))
         return GX_ERR_BAD_MEM_ALLOC;
@@ -410,7 +413,7 @@ int grow(T **p, size_t &cap, size_t n, uint64_t &gen) {
     return st;
 }
 
-constexpr uint64_t kPosChunkWords = 1ull << 27;  // 512 MB of slice positions per planning chunk
+constexpr uint64_t kPosChunkWords = 1ull << 29;  // 2 GB of slice positions per planning chunk
 
 struct Batched {
     int cfg_id;
@@ -451,7 +454,6 @@ Batched setup_batched(gx_ctx *c, const gx_dindex *idx, gx_dqueries *q, const gx_
     pa.pair_hi = q->pair_hi;
     pa.pair_tiles = q->pair_tiles;
     pa.pair_pack = q->pair_pack;
-    pa.pair_item_off = q->pair_item_off;
     pa.pair_pos_off = q->pair_pos_off;
     pa.n_items = q->n_items;
     pa.n_pos = q->n_pos;
@@ -471,6 +473,7 @@ void fill_item_args(Batched &B, gx_ctx *c, const gx_dindex *idx, gx_dqueries *q,
     ia.qidx = q->idx;
     ia.dF = p->dF;
     ia.min_shp = p->min_shp;
+    ia.nbins = (int)idx->nbins;
     ia.lgfact = c->d_lgfact;
     ia.lgfact_n = c->lgfact_n;
     ia.lq = q->lq;
@@ -483,25 +486,28 @@ void fill_item_args(Batched &B, gx_ctx *c, const gx_dindex *idx, gx_dqueries *q,
     ia.pair_pos_off = q->pair_pos_off;
     ia.pos_off = q->pos_off;
     ia.pos = q->pos;
-    ia.items = q->items;
+    ia.order = q->order;
     ia.cand_scratch = c->d_cand;
     ia.tile_entries_max = B.cfg.tile_words * 4;
     ia.split = B.split;
+    ia.max_speclen = idx->max_speclen;
+    ia.smax_m = idx->smax_m;
+    ia.smax_s = idx->smax_s;
+    ia.cell_shift = idx->cell_shift;
     ia.hist = q->hist;
     ia.cpsms = q->cpsms;
-    ia.item_key = q->item_key;
-    ia.item_eq = q->item_eq;
-    ia.item_shp = q->item_shp;
+    ia.best_key = q->best_key;
+    ia.best_eq = q->best_eq;
+    ia.best_shp = q->best_shp;
     ia.stats = c->d_stats;
     EvalArgs &ea = B.ea;
     ea.buckets = idx->d_buckets;
     ea.n_spectra = q->n_spectra;
     ea.min_cpsm = p->min_cpsm;
     ea.expect_max = p->expect_max;
-    ea.item_off = q->item_off;
-    ea.item_key = q->item_key;
-    ea.item_eq = q->item_eq;
-    ea.item_shp = q->item_shp;
+    ea.best_key = q->best_key;
+    ea.best_eq = q->best_eq;
+    ea.best_shp = q->best_shp;
     ea.hist = q->hist;
     ea.cpsms = q->cpsms;
     ea.route = nullptr;
@@ -515,8 +521,6 @@ hipError_t enqueue_prefix(gx_ctx *c, gx_dqueries *q, const Batched &B, hipStream
     hipError_t e;
     if ((e = launch_fill_u32(reinterpret_cast<uint32_t *>(c->d_stats), (uint64_t)kStatRows * ST_N * 2, 0, sm)))
         return e;
-    if ((e = launch_fill_u32(q->hist, (uint64_t)q->n_spectra * GX_HIST_BINS, 0, sm))) return e;
-    if ((e = launch_fill_u32(q->cpsms, q->n_spectra, 0, sm))) return e;
     if ((e = launch_plan_spectra(B.pa, sm))) return e;
     return launch_scan(B.pa, sm);
 }
@@ -528,18 +532,17 @@ hipError_t enqueue_chunks(gx_ctx *c, gx_dqueries *q, Batched &B, hipStream_t sm,
     const size_t n_chunks = q->bounds.size() - 1;
     if ((e = launch_fill_u32(q->counters, std::max<size_t>(64, n_chunks), 0, sm))) return e;
     B.pa.pos = q->pos;
-    B.pa.items = q->items;
     for (size_t k = 0; k < n_chunks; k++) {
         B.pa.chunk_s0 = q->bounds[k];
         B.pa.chunk_s1 = q->bounds[k + 1];
         if (events && (e = hipEventRecord(c->ev_chunk[2 * k], sm))) return e;
         if ((e = launch_plan_slices(B.pa, sm))) return e;
         if (events && (e = hipEventRecord(c->ev_chunk[2 * k + 1], sm))) return e;
-        B.ia.item0 = q->h_item_off[k];
-        B.ia.item1 = q->h_item_off[k + 1];
         B.ia.chunk_s0 = q->bounds[k];
+        B.ia.chunk_s1 = q->bounds[k + 1];
         B.ia.work_counter = q->counters + k;
-        if (B.ia.item1 > B.ia.item0 && (e = launch_search_items(B.cfg_id, B.ia, B.n_blocks, sm))) return e;
+        // every spectrum's results are written by the search kernel
+        if ((e = launch_search_spectra(B.cfg_id, B.ia, B.n_blocks, sm))) return e;
     }
     if (events && (e = hipEventRecord(c->ev2, sm))) return e;
     if (B.learned) return enqueue_learned_evalue(c, q, B.ea, B.conf, sm);
@@ -608,21 +611,25 @@ int search_batched(gx_ctx *c, const gx_dindex *idx, gx_dqueries *q, const gx_sea
         }
     }
     const size_t n_chunks = q->bounds.size() - 1;
-    if (grow(&q->pos, q->pos_cap, max_chunk, q->buf_gen) ||
-        grow(&q->items, q->items_cap, 2 * (size_t)n_items_total + 2, q->buf_gen))
-        return GX_ERR_BAD_MEM_ALLOC;
-    if (q->items_cap / 2 + 1 > q->slots_cap || !q->item_key) {
-        q->buf_gen++;
-        // per-item slots follow the item capacity
-        if (q->item_key) (void)hipFree(q->item_key);
-        if (q->item_eq) (void)hipFree(q->item_eq);
-        if (q->item_shp) (void)hipFree(q->item_shp);
-        q->item_key = nullptr;
-        q->item_eq = q->item_shp = nullptr;
-        const size_t n = q->items_cap / 2 + 1;
-        if (dalloc(&q->item_key, n) || dalloc(&q->item_eq, n) || dalloc(&q->item_shp, n)) return GX_ERR_BAD_MEM_ALLOC;
-        q->slots_cap = n;
+    // Spectra are handed out largest first within each chunk (the slice
+    // positions a spectrum needs stand in for its work), so that the chunk
+    // ends on small spectra: a counting sort by the bit length of the work.
+    q->h_order.resize(ns);
+    for (size_t k = 0; k < n_chunks; k++) {
+        const uint32_t s0 = q->bounds[k], s1 = q->bounds[k + 1];
+        uint32_t cnt[66] = {};
+        auto cls = [&](uint32_t s) { return 64 - __builtin_clzll(q->h_pos_off[s + 1] - q->h_pos_off[s] + 1); };
+        for (uint32_t s = s0; s < s1; s++) cnt[65 - cls(s)]++;
+        uint32_t run = s0;
+        for (int c = 0; c < 66; c++) {
+            const uint32_t n = cnt[c];
+            cnt[c] = run;
+            run += n;
+        }
+        for (uint32_t s = s0; s < s1; s++) q->h_order[cnt[65 - cls(s)]++] = s;
     }
+    if (ns) GX_HIP(hipMemcpyAsync(q->order, q->h_order.data(), sizeof(uint32_t) * ns, hipMemcpyHostToDevice, sm));
+    if (grow(&q->pos, q->pos_cap, max_chunk, q->buf_gen)) return GX_ERR_BAD_MEM_ALLOC;
     if (!q->counters || n_chunks > q->counters_n) {
         q->buf_gen++;
         if (q->counters) (void)hipFree(q->counters);
@@ -651,11 +658,6 @@ int search_batched(gx_ctx *c, const gx_dindex *idx, gx_dqueries *q, const gx_sea
         GX_HIP(hipEventCreate(&e));
         c->ev_chunk.push_back(e);
     }
-    q->h_item_off.assign(n_chunks + 1, 0);
-    for (size_t k = 0; k <= n_chunks; k++)
-        GX_HIP(hipMemcpyAsync(&q->h_item_off[k], q->item_off + q->bounds[k], sizeof(uint32_t), hipMemcpyDeviceToHost,
-                              sm));
-    GX_HIP(hipStreamSynchronize(sm));
     q->n_items_total = n_items_total;
     fill_item_args(B, c, idx, q, p);
     GX_HIP(enqueue_chunks(c, q, B, sm, true));

@@ -49,6 +49,8 @@ struct gx_dindex {
     int maxz = 1;
     uint32_t nbins = 0;
     uint32_t max_speclen = 0;
+    uint32_t smax_m = 0, smax_s = 0;  // exact division by max_speclen
+    uint32_t cell_shift = 7;         // coarse peak-index cell: 2^cell_shift bins
     std::vector<void *> allocs;
     DevBucket *d_buckets = nullptr;
     std::vector<DevBucket> h_buckets;
@@ -73,22 +75,21 @@ struct gx_dqueries {
     int32_t *lq = nullptr, *li = nullptr;
     uint32_t *n_legal = nullptr;
     int32_t *pair_lo = nullptr, *pair_hi = nullptr;
-    uint32_t *pair_tiles = nullptr, *pair_pack = nullptr, *pair_item_off = nullptr, *pair_pos_off = nullptr;
+    uint32_t *pair_tiles = nullptr, *pair_pack = nullptr, *pair_pos_off = nullptr;
     uint32_t *n_items = nullptr, *item_off = nullptr;
     uint64_t *n_pos = nullptr, *pos_off = nullptr;
     uint32_t *cpsms = nullptr;
     uint32_t *counters = nullptr;  // one work counter per chunk
     size_t counters_n = 0;
-    size_t slots_cap = 0;          // per-item top-hit slots
-    uint32_t *items = nullptr;
-    size_t items_cap = 0;  // items
-    unsigned long long *item_key = nullptr;
-    uint32_t *item_eq = nullptr, *item_shp = nullptr;
+    unsigned long long *best_key = nullptr;  // per-spectrum top hit of the search kernel
+    uint32_t *best_eq = nullptr, *best_shp = nullptr;
+    uint32_t *order = nullptr;             // spectra of each chunk, largest first
+    std::vector<uint32_t> h_order;
     uint32_t *pos = nullptr;
     size_t pos_cap = 0;  // words
     std::vector<uint64_t> h_pos_off;
     // chunking of the last eager run, and the graph captured from it
-    std::vector<uint32_t> bounds, h_item_off;
+    std::vector<uint32_t> bounds;
     uint32_t n_items_total = 0;
     hipGraph_t graph = nullptr;
     hipGraphExec_t graph_exec = nullptr;
