@@ -446,6 +446,7 @@ __global__ void __launch_bounds__(THREADS, WAVES_PER_SIMD) search_spectra_kernel
             const uint32_t *pos = a.pos + pos_s + a.pair_pos_off[pi];
             const auto ia = gptr(bk.iA);
             const uint32_t dm = bk.div_m, dsh = bk.div_s;
+            const uint32_t btag = b << 26;
             for (uint32_t t = 0; t < tiles; t++) {
                 const int32_t tlo = lo + (int32_t)(t * T);
                 const int32_t thi = min(hi, tlo + (int32_t)T - 1);
@@ -463,14 +464,16 @@ __global__ void __launch_bounds__(THREADS, WAVES_PER_SIMD) search_spectra_kernel
                     static_assert(SL == THREADS, "one slice per thread");
                     uint32_t at = 0, len = 0;
                     if (tid < ns) {
+                        // all four words at once (one round trip): a small bin is
+                        // [start | flag, end), a larger one has tile boundaries
                         const uint32_t *ps = pos + (uint64_t)(c0 + tid) * stride;
-                        const uint32_t p0 = ps[0];
+                        const uint32_t p0 = ps[0], p1 = ps[1], pt = ps[t], pt1 = ps[t + 1];
                         if (p0 & kWholeFlag) {  // small bin: the whole bin, filtered by id below
                             at = p0 & ~kWholeFlag;
-                            len = ps[1] - at;
+                            len = p1 - at;
                         } else {
-                            at = ps[t];
-                            len = ps[t + 1] - at;
+                            at = pt;
+                            len = pt1 - at;
                         }
                     }
                     const uint32_t fl = len ? 1u : 0u;
@@ -551,8 +554,19 @@ __global__ void __launch_bounds__(THREADS, WAVES_PER_SIMD) search_spectra_kernel
                                 if (raw[u] - tlo_id < span_id) {
                                     const uint32_t el = (__umulhi(raw[u], dm) >> dsh) - (uint32_t)tlo;
                                     if (el < nent) {
-                                        atomicAdd(&L.cnt[el >> lp], 1u << ((el & (pack - 1)) << fsh));
+                                        const uint32_t sh = (el & (pack - 1)) << fsh;
+                                        const uint32_t old = atomicAdd(&L.cnt[el >> lp], 1u << sh);
                                         n_scanned++;
+                                        // the increment that reaches min_shp makes the entry a
+                                        // candidate, exactly once
+                                        if (((old >> sh) & fmask) == min_cnt - 1u) {
+                                            const uint32_t p = atomicAdd(&L.n_cand, 1u);
+                                            const uint32_t cd = btag | ((uint32_t)tlo + el);
+                                            if (p < (uint32_t)kCandLdsSpec)
+                                                L.cand[p] = cd;
+                                            else
+                                                g_cand[p - kCandLdsSpec] = cd;
+                                        }
                                     } else {
                                         n_errors++;  // never expected (exact division is checked on the host)
                                     }
@@ -563,43 +577,11 @@ __global__ void __launch_bounds__(THREADS, WAVES_PER_SIMD) search_spectra_kernel
                     }
                 }
                 __syncthreads();
-                // ---- candidates: entries with count >= min_shp; clear the tile ----
+                // ---- the candidates were collected in pass 1; clear the tile ----
                 {
-                    // SWAR "some counter >= min_cnt" test per word (min_cnt <= 128)
-                    const uint32_t ones = pack == 4 ? 0x01010101u : (pack == 2 ? 0x00010001u : 1u);
-                    const uint32_t hib = ones << (fbits - 1);
-                    const uint32_t low = pack == 1 ? 0x7fffffffu : (~hib);
-                    const bool swar = min_cnt <= 128;
-                    const uint32_t addv = ((1u << (fbits - 1)) - min_cnt) * ones;
-                    const uint32_t nw = (nent + pack - 1) >> lp;
-                    const uint32_t nq = (nw + 3) >> 2;
-                    const uint32_t btag = b << 26;
+                    const uint32_t nq = (((nent + pack - 1) >> lp) + 3) >> 2;
                     uint4 *cq = reinterpret_cast<uint4 *>(L.cnt);
-                    for (uint32_t q = tid; q < nq; q += THREADS) {
-                        const uint4 v4 = cq[q];
-                        if ((v4.x | v4.y | v4.z | v4.w) == 0) continue;
-                        cq[q] = make_uint4(0, 0, 0, 0);
-                        const uint32_t vv[4] = {v4.x, v4.y, v4.z, v4.w};
-#pragma unroll
-                        for (int i = 0; i < 4; i++) {
-                            const uint32_t v = vv[i];
-                            if (v == 0) continue;
-                            if (swar && ((((v & low) + addv) | v) & hib) == 0) continue;
-                            const uint32_t w = 4 * q + i;
-                            for (uint32_t e = 0; e < pack; e++) {
-                                const uint32_t cc = (v >> (e * fbits)) & fmask;
-                                const uint32_t el = (w << lp) + e;
-                                if (cc >= min_cnt && el < nent) {
-                                    const uint32_t p = atomicAdd(&L.n_cand, 1u);
-                                    const uint32_t cd = btag | ((uint32_t)tlo + el);
-                                    if (p < (uint32_t)kCandLdsSpec)
-                                        L.cand[p] = cd;
-                                    else
-                                        g_cand[p - kCandLdsSpec] = cd;
-                                }
-                            }
-                        }
-                    }
+                    for (uint32_t q = tid; q < nq; q += THREADS) cq[q] = make_uint4(0, 0, 0, 0);
                 }
             }
         }
