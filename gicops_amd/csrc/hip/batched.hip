@@ -263,8 +263,8 @@ struct SpecLds {
     uint32_t s_flat[SL + 1];   // exclusive prefix of their lengths
     uint32_t wsum[THREADS / 64];
     uint32_t wsum2[THREADS / 64];
-    uint16_t cj0[THREADS];     // slice of each chunk's first id, per round
-    unsigned long long cmask[THREADS];  // lanes of each chunk where a later slice starts
+    uint16_t cj0[THREADS];     // slices starting before each chunk of the round
+    unsigned long long cmask[THREADS];  // lanes of each chunk where a slice starts
     uint32_t ne;
     uint32_t cand[kCandLdsSpec];  // the spectrum's candidates, b << 26 | psid (the rest in global scratch)
     uint32_t hist[GX_HIST_BINS];
@@ -429,12 +429,24 @@ __global__ void __launch_bounds__(THREADS, WAVES_PER_SIMD) search_spectra_kernel
         uint32_t beq = 0, bshp = 0, bcps = 0;
         uint32_t n_scanned = 0, n_errors = 0;
         const uint64_t pos_s = a.pos_off[s] - a.pos_off[a.chunk_s0];
-        for (uint32_t b = 0; b < nbk; b++) {
-            const uint32_t pi = s * nbk + b;
-            const uint32_t tiles = a.pair_tiles[pi];
-            if (tiles == 0) continue;
-            const int32_t lo = a.pair_lo[pi], hi = a.pair_hi[pi];
-            const uint32_t pack = a.pair_pack[pi];
+        // the spectrum's per-bucket plan, bucket b in lane b of every wave (one
+        // round trip per spectrum instead of a chain per item)
+        uint32_t w_tiles = 0, w_pack = 0, w_poff = 0;
+        int32_t w_lo = 0, w_hi = -1;
+        if ((uint32_t)lane < nbk) {
+            const uint32_t pi = s * nbk + (uint32_t)lane;
+            w_tiles = a.pair_tiles[pi];
+            w_pack = a.pair_pack[pi];
+            w_poff = a.pair_pos_off[pi];
+            w_lo = a.pair_lo[pi];
+            w_hi = a.pair_hi[pi];
+        }
+        for (unsigned long long todo = __ballot(w_tiles > 0); todo; todo &= todo - 1) {
+            const uint32_t b = (uint32_t)__builtin_ctzll(todo);
+            const uint32_t tiles = (uint32_t)__builtin_amdgcn_readlane((int)w_tiles, (int)b);
+            const int32_t lo = __builtin_amdgcn_readlane(w_lo, (int)b), hi = __builtin_amdgcn_readlane(w_hi, (int)b);
+            const uint32_t pack = (uint32_t)__builtin_amdgcn_readlane((int)w_pack, (int)b);
+            const uint32_t poff = (uint32_t)__builtin_amdgcn_readlane((int)w_poff, (int)b);
             const uint32_t lp = pack == 4 ? 2u : (pack == 2 ? 1u : 0u);  // log2 entries per word
             const uint32_t fbits = 32u >> lp;                             // counter width
             const uint32_t fmask = fbits == 32 ? 0xffffffffu : ((1u << fbits) - 1u);
@@ -443,7 +455,7 @@ __global__ void __launch_bounds__(THREADS, WAVES_PER_SIMD) search_spectra_kernel
             const DevBucket bk = a.buckets[b];
             const uint32_t speclen = bk.speclen;
             const uint32_t stride = tiles + 1;
-            const uint32_t *pos = a.pos + pos_s + a.pair_pos_off[pi];
+            const uint32_t *pos = a.pos + pos_s + poff;
             const auto ia = gptr(bk.iA);
             const uint32_t dm = bk.div_m, dsh = bk.div_s;
             const uint32_t btag = b << 26;
@@ -498,25 +510,36 @@ __global__ void __launch_bounds__(THREADS, WAVES_PER_SIMD) search_spectra_kernel
                     }
                     __syncthreads();
                     // the tile's ions, flattened over the non-empty slices, in rounds
-                    // of THREADS 64-id chunks. One thread per chunk finds the slice
-                    // holding the chunk's first id and a bit mask of the lanes where
-                    // later slices start.
+                    // of THREADS 64-id chunks. Each slice sets the bit of its first
+                    // lane in its chunk's mask; a scan of the masks' popcounts gives
+                    // the number of slices starting before each chunk.
                     const uint32_t ne = L.ne;
                     const uint32_t N = L.s_flat[ne];
                     const uint32_t nch = (N + 63) >> 6;
+                    uint32_t base = 0;  // slices starting before the round
                     for (uint32_t r0 = 0; r0 < nch; r0 += THREADS) {
                         const uint32_t nr = min((uint32_t)THREADS, nch - r0);
-                        if ((uint32_t)tid < nr) {
-                            const uint32_t f0 = (r0 + tid) << 6;
-                            const uint32_t j = (uint32_t)upper_bound_lds(L.s_flat, (int)ne + 1, f0) - 1;
-                            unsigned long long mk = 0;
-                            for (uint32_t k = j + 1; k < ne; k++) {
-                                const uint32_t d = L.s_flat[k] - f0;
-                                if (d >= 64) break;
-                                mk |= 1ull << d;
+                        if ((uint32_t)tid < nr) L.cmask[tid] = 0;
+                        __syncthreads();
+                        if ((uint32_t)tid < ne) {
+                            const uint32_t f = L.s_flat[tid];
+                            const uint32_t c = (f >> 6) - r0;
+                            if (f >= (r0 << 6) && c < nr) atomicOr(&L.cmask[c], 1ull << (f & 63u));
+                        }
+                        __syncthreads();
+                        {
+                            const uint32_t v = (uint32_t)tid < nr ? (uint32_t)__popcll(L.cmask[tid]) : 0u;
+                            const uint32_t inc = wave_incl_scan(v, lane);
+                            if (lane == 63) L.wsum[wave] = inc;
+                            __syncthreads();
+                            uint32_t wb = 0, tot = 0;
+                            for (int w = 0; w < NW; w++) {
+                                const uint32_t x = L.wsum[w];
+                                wb += w < wave ? x : 0u;
+                                tot += x;
                             }
-                            L.cj0[tid] = (uint16_t)j;
-                            L.cmask[tid] = mk;
+                            if ((uint32_t)tid < nr) L.cj0[tid] = (uint16_t)(base + wb + inc - v);
+                            base += tot;
                         }
                         __syncthreads();
                         // U chunks per wave in three batched LDS/HBM stages: chunk
@@ -536,7 +559,8 @@ __global__ void __launch_bounds__(THREADS, WAVES_PER_SIMD) search_spectra_kernel
                             }
 #pragma unroll
                             for (int u = 0; u < U; u++) {
-                                const uint32_t j = j0[u] + (uint32_t)__popcll(mk[u] & upto);
+                                // slices starting at or before this lane, minus one
+                                const uint32_t j = j0[u] + (uint32_t)__popcll(mk[u] & upto) - 1u;
                                 sa[u] = L.s_a[j];
                                 sf[u] = L.s_flat[j];
                             }
