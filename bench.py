@@ -56,16 +56,21 @@ def algorithmic_bytes(st: dict, n_peaks: int) -> int:
     return 8 * n_peaks + 8 * st["bins"] + 4 * st["ions"] + 24 * st["spectra"]
 
 
-def traffic_from_profiles(workload: str):
+def traffic_from_profiles(workload: str, config: int, whole_bin: int):
     """HBM bytes per launch of the search kernel from the committed rocprofv3
-    PMC summary (FETCH_SIZE and WRITE_SIZE, scaled by a copy calibration)."""
-    f = ROOT / "profiles" / f"search_items_{workload}_hbm_traffic.txt"
+    PMC summary (FETCH_SIZE and WRITE_SIZE, scaled by a copy calibration),
+    when it was taken with the same launch configuration."""
+    f = ROOT / "profiles" / f"search_spectra_{workload}_hbm_traffic.txt"
     if not f.exists():
         return None
+    vals = {}
     for line in f.read_text().splitlines():
-        if line.startswith("hbm_bytes_per_launch:"):
-            return float(line.split(":", 1)[1])
-    return None
+        if ":" in line and not line.startswith("#"):
+            k, v = line.split(":", 1)
+            vals[k.strip()] = v.strip()
+    if int(vals.get("launch_config", -1)) != config or int(vals.get("whole_bin", -1)) != whole_bin:
+        return None
+    return float(vals["hbm_bytes_per_launch"]) if "hbm_bytes_per_launch" in vals else None
 
 
 def cpu_baseline(index, batch, params_ffi, budget_s: float):
@@ -200,8 +205,9 @@ def main() -> int:
     if args.calibrate_copy:
         return calibrate_copy(args.calibrate_copy)
     if args.gpus != 1:
-        print(f"bench.py: this build runs on one GPU only (asked for {args.gpus}); the search shards by "
-              "spectrum with a replicated index, see DESIGN.md", file=sys.stderr)
+        print(f"bench.py: this build runs on one GPU only (asked for {args.gpus}); on more GPUs the search "
+              "would shard by spectrum over a replicated index, with no communication until results are "
+              "gathered", file=sys.stderr)
         return 2
 
     ensure_built()
@@ -307,9 +313,9 @@ def main() -> int:
         cpu = cpu_baseline(index, batch, O.default_params(dM=dM), args.cpu_budget)
         log(f"cpu baseline: {cpu}")
 
-    # the committed counter summary covers the batched kernel in configuration 1
-    traffic = traffic_from_profiles(args.workload) if (engine == F.GX_ENGINE_BATCHED and eager["config"] == 1) \
-        else None
+    whole = args.whole_bin or extra.get("autotune", {}).get("whole_bin", 64)
+    traffic = (traffic_from_profiles(args.workload, eager["config"], whole) if engine == F.GX_ENGINE_BATCHED
+               else None)
     out = {
         "metric": "spectra/sec, 2M-peptide open search: learned+batched vs plain tail-fit, 1 GPU",
         "value": round(value, 1),
@@ -345,7 +351,7 @@ def main() -> int:
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
             "traffic": traffic,
-            "kernel": "search_items" if engine == F.GX_ENGINE_BATCHED else "search_kernel",
+            "kernel": "search_spectra" if engine == F.GX_ENGINE_BATCHED else "search_kernel",
             "launches": launches,
             "avg_launch_ms": round(avg_kms / launches, 4),
             "algorithmic_bytes_per_launch": round(4 * st["ions"] / launches),
