@@ -1,0 +1,88 @@
+"""The batched engine at the benchmark's full sizes, on the 2M-peptide open
+index (2M tryptic peptides + Met oxidation, ~5.6M entries, ~290M ions):
+
+- the open workload (100k spectra, 50-150 peaks, dM +-500 Da) and the uneven
+  workload (200k spectra, 20-2000 peaks, dM drawn from {10, 100, 500} Da per
+  spectrum), each searched in one call (planning chunks, one persistent
+  launch per chunk) and replayed from its hipGraph;
+- a seeded random sample of spectra checked against the CPU oracle to the
+  parity bar (bit-exact counts, candidates, histograms and top hits);
+- size-independent properties over every spectrum: each histogram sums to the
+  spectrum's candidate count, the top hit's bin is occupied, graph replays are
+  bit-identical to the eager run, the work counters equal the oracle's on the
+  sample, and the e-value is cpsms * 10^(mu/1e6 * hyp + beta/1e6).
+"""
+import numpy as np
+import pytest
+
+from gicops_amd import _ffi as F
+from gicops_amd import slm
+from gicops_amd.search import Searcher, SearchParams
+from oracle import pyoracle as O
+
+from test_gpu_parity import compare  # noqa: E402
+
+pytestmark = pytest.mark.gpu
+
+SAMPLE = 300
+
+
+def say(msg):
+    print(msg, flush=True)
+
+
+@pytest.fixture(scope="module")
+def big_index():
+    say("building the 2M-peptide open index")
+    return slm.SlmIndex(slm.DB_PRESETS["open2m"])
+
+
+@pytest.fixture(scope="module")
+def big_searcher(big_index):
+    s = Searcher(big_index)
+    yield s
+    s.close()
+
+
+def _invariants(psm, hist):
+    assert np.array_equal(hist.sum(axis=1, dtype=np.int64), psm["cpsms"].astype(np.int64))
+    hit = (psm["flags"] & F.GX_PSM_HAS_HIT) != 0
+    assert np.all(hist[np.nonzero(hit)[0], psm["hyp"][hit]] > 0)
+    assert np.all(psm["psid"][~hit] == -1)
+    fit = (psm["flags"] & F.GX_PSM_FIT_OK) != 0
+    e = psm["cpsms"][fit] * np.power(10.0, psm["mu"][fit] / 1e6 * psm["hyp"][fit] + psm["beta"][fit] / 1e6)
+    assert np.allclose(e, psm["evalue"][fit], rtol=1e-12, atol=0)
+    assert np.all(np.isinf(psm["evalue"][~fit]))
+    rep = (psm["flags"] & F.GX_PSM_REPORTED) != 0
+    assert np.all(psm["evalue"][rep] < 20.0) and np.all(~rep | fit)
+
+
+@pytest.mark.parametrize("workload", ["open2m", "uneven"])
+def test_full_size_workload(big_index, big_searcher, workload):
+    sc = slm.SPEC_PRESETS[workload]
+    say(f"case: full-size {workload}: {sc.n_spectra} spectra")
+    batch = slm.generate_spectra(big_index, sc)
+    dq = big_searcher.upload(batch)
+    p = SearchParams(dM=500.0)
+    st = big_searcher.search_device(dq, p)
+    psm, hist = big_searcher.download(dq, want_hist=True)
+    say(f"  eager: {st['total_ms']:.1f} ms, {st['chunks']} chunks, errors {st['errors']}")
+    assert st["spectra"] == batch.n_spectra and st["errors"] == 0 and st["ions_scanned"] == st["ions"]
+    _invariants(psm, hist)
+    # graph capture, then two replays: identical to the eager results
+    g = SearchParams(dM=500.0, use_graph=True)
+    big_searcher.search_device(dq, g)
+    for _ in range(2):
+        big_searcher.search_device(dq, g)
+        p2, h2 = big_searcher.download(dq, want_hist=True)
+        assert p2.tobytes() == psm.tobytes() and np.array_equal(h2, hist)
+    # a seeded sample against the oracle, with the work counters
+    rng = np.random.default_rng(2024)
+    idx = np.sort(rng.choice(batch.n_spectra, SAMPLE, replace=False))
+    sub = batch.subset(idx)
+    c, hc, cnt = O.search(big_index, sub, O.default_params(dM=500.0), want_hist=True, want_counters=True)
+    compare(psm[idx], c, hist[idx], hc, f"full-size {workload} sample")
+    gs, _, gst = big_searcher.search(sub, SearchParams(dM=500.0))
+    assert gst["ions"] == int(cnt["ions"].sum()) and gst["window"] == int(cnt["window"].sum())
+    assert gs.tobytes() == psm[idx].tobytes()
+    dq.free()
