@@ -303,7 +303,8 @@ __device__ __forceinline__ void merge_best(unsigned long long &k, uint32_t &e, u
 // 2^cell_shift-bin cell) kept in the tail of the count tile.
 template <int THREADS, int TILE_WORDS, int SL>
 __device__ void rescore_candidates(const ItemArgs &a, SpecLds<THREADS, TILE_WORDS, SL> &L, int P, uint32_t *g_cand,
-                                   unsigned long long &bkey, uint32_t &beq, uint32_t &bshp, uint32_t &bcps) {
+                                   unsigned long long &bkey, uint32_t &beq, uint32_t &bshp, uint32_t &bcps,
+                                   uint32_t &n_err) {
     constexpr int CB = (TILE_WORDS - kPeakIdxWords) / 3;  // candidates per round (3 accumulators each)
     const int tid = threadIdx.x;
     const int dF = a.dF;
@@ -366,6 +367,9 @@ __device__ void rescore_candidates(const ItemArgs &a, SpecLds<THREADS, TILE_WORD
             const uint32_t ci = c0 + c;
             const uint32_t cd = ci < (uint32_t)kCandLdsSpec ? L.cand[ci] : gptr(g_cand)[ci - kCandLdsSpec];
             const uint32_t shpk = bc + yc;
+            // pass 1 counted at least min_shp matches for this entry; the exact
+            // recount must agree
+            n_err += shpk < (uint32_t)max(a.min_shp, 1) ? 1u : 0u;
             if (shpk >= (uint32_t)a.min_shp && (int)bc < a.lgfact_n && (int)yc < a.lgfact_n) {
                 float hs = hyperscore(bc, yc, ibc, iyc, a.lgfact);
                 if (hs > 0) {
@@ -402,6 +406,7 @@ __global__ void __launch_bounds__(THREADS, WAVES_PER_SIMD) search_spectra_kernel
     const uint32_t cand_cap = (uint32_t)kCandLdsSpec + (uint32_t)a.tile_entries_max;
     for (int w = tid; w < TILE_WORDS; w += THREADS) L.cnt[w] = 0;
     for (int w = tid; w < GX_HIST_BINS; w += THREADS) L.hist[w] = 0;
+    L.cmask[tid] = 0;
     for (int b = tid; b < a.n_buckets; b += THREADS) L.b_speclen[b] = a.buckets[b].speclen;
     const uint32_t nspec = a.chunk_s1 - a.chunk_s0;
     if (tid == 0) {
@@ -468,7 +473,7 @@ __global__ void __launch_bounds__(THREADS, WAVES_PER_SIMD) search_spectra_kernel
                 // room for every entry of this tile as a candidate
                 __syncthreads();
                 if (L.n_cand + nent > cand_cap)
-                    rescore_candidates<THREADS, TILE_WORDS, SL>(a, L, P, g_cand, bkey, beq, bshp, bcps);
+                    rescore_candidates<THREADS, TILE_WORDS, SL>(a, L, P, g_cand, bkey, beq, bshp, bcps, n_errors);
                 // ---- pass 1: shared-peak counts of every entry in the tile ----
                 for (int c0 = 0; c0 < nsl; c0 += SL) {
                     const int ns = min(SL, nsl - c0);
@@ -500,33 +505,37 @@ __global__ void __launch_bounds__(THREADS, WAVES_PER_SIMD) search_spectra_kernel
                         bl += L.wsum[w];
                         bf += L.wsum2[w];
                     }
+                    // the tile's ions, flattened over the non-empty slices, in rounds
+                    // of THREADS 64-id chunks. Each slice sets the bit of its first
+                    // lane in its chunk's mask (for the first round, right here); a
+                    // scan of the masks' popcounts gives the number of slices
+                    // starting before each chunk. The masks are clear on entry.
                     if (fl) {
+                        const uint32_t f = bl + il - len;
                         L.s_a[bf + ifl - 1] = at;
-                        L.s_flat[bf + ifl - 1] = bl + il - len;
+                        L.s_flat[bf + ifl - 1] = f;
+                        if ((f >> 6) < (uint32_t)THREADS) atomicOr(&L.cmask[f >> 6], 1ull << (f & 63u));
                     }
                     if (tid == THREADS - 1) {
                         L.s_flat[bf + ifl] = bl + il;
                         L.ne = bf + ifl;
                     }
                     __syncthreads();
-                    // the tile's ions, flattened over the non-empty slices, in rounds
-                    // of THREADS 64-id chunks. Each slice sets the bit of its first
-                    // lane in its chunk's mask; a scan of the masks' popcounts gives
-                    // the number of slices starting before each chunk.
                     const uint32_t ne = L.ne;
                     const uint32_t N = L.s_flat[ne];
                     const uint32_t nch = (N + 63) >> 6;
                     uint32_t base = 0;  // slices starting before the round
                     for (uint32_t r0 = 0; r0 < nch; r0 += THREADS) {
                         const uint32_t nr = min((uint32_t)THREADS, nch - r0);
-                        if ((uint32_t)tid < nr) L.cmask[tid] = 0;
-                        __syncthreads();
-                        if ((uint32_t)tid < ne) {
-                            const uint32_t f = L.s_flat[tid];
-                            const uint32_t c = (f >> 6) - r0;
-                            if (f >= (r0 << 6) && c < nr) atomicOr(&L.cmask[c], 1ull << (f & 63u));
+                        if (r0 > 0) {  // later rounds: the masks were cleared after the previous one
+                            __syncthreads();
+                            if ((uint32_t)tid < ne) {
+                                const uint32_t f = L.s_flat[tid];
+                                const uint32_t c = (f >> 6) - r0;
+                                if (f >= (r0 << 6) && c < nr) atomicOr(&L.cmask[c], 1ull << (f & 63u));
+                            }
+                            __syncthreads();
                         }
-                        __syncthreads();
                         {
                             const uint32_t v = (uint32_t)tid < nr ? (uint32_t)__popcll(L.cmask[tid]) : 0u;
                             const uint32_t inc = wave_incl_scan(v, lane);
@@ -573,31 +582,29 @@ __global__ void __launch_bounds__(THREADS, WAVES_PER_SIMD) search_spectra_kernel
                             }
 #pragma unroll
                             for (int u = 0; u < U; u++) {
-                                // ids of the tile's entries: [tlo_id, tlo_id + span_id), so
-                                // el < nent follows; the guard only keeps LDS writes in the tile
+                                // ids of the tile's entries are [tlo_id, tlo_id + span_id);
+                                // with the exact division (checked on the host for every
+                                // id < 2^31) that gives 0 <= el < nent
                                 if (raw[u] - tlo_id < span_id) {
                                     const uint32_t el = (__umulhi(raw[u], dm) >> dsh) - (uint32_t)tlo;
-                                    if (el < nent) {
-                                        const uint32_t sh = (el & (pack - 1)) << fsh;
-                                        const uint32_t old = atomicAdd(&L.cnt[el >> lp], 1u << sh);
-                                        n_scanned++;
-                                        // the increment that reaches min_shp makes the entry a
-                                        // candidate, exactly once
-                                        if (((old >> sh) & fmask) == min_cnt - 1u) {
-                                            const uint32_t p = atomicAdd(&L.n_cand, 1u);
-                                            const uint32_t cd = btag | ((uint32_t)tlo + el);
-                                            if (p < (uint32_t)kCandLdsSpec)
-                                                L.cand[p] = cd;
-                                            else
-                                                g_cand[p - kCandLdsSpec] = cd;
-                                        }
-                                    } else {
-                                        n_errors++;  // never expected (exact division is checked on the host)
+                                    const uint32_t sh = (el & (pack - 1)) << fsh;
+                                    const uint32_t old = atomicAdd(&L.cnt[el >> lp], 1u << sh);
+                                    n_scanned++;
+                                    // the increment that reaches min_shp makes the entry a
+                                    // candidate, exactly once
+                                    if (((old >> sh) & fmask) == min_cnt - 1u) {
+                                        const uint32_t p = atomicAdd(&L.n_cand, 1u);
+                                        const uint32_t cd = btag | ((uint32_t)tlo + el);
+                                        if (p < (uint32_t)kCandLdsSpec)
+                                            L.cand[p] = cd;
+                                        else
+                                            g_cand[p - kCandLdsSpec] = cd;
                                     }
                                 }
                             }
                         }
                         __syncthreads();
+                        L.cmask[tid] = 0;  // ordered before the next use by at least one barrier
                     }
                 }
                 __syncthreads();
@@ -610,7 +617,7 @@ __global__ void __launch_bounds__(THREADS, WAVES_PER_SIMD) search_spectra_kernel
             }
         }
         __syncthreads();
-        if (L.n_cand) rescore_candidates<THREADS, TILE_WORDS, SL>(a, L, P, g_cand, bkey, beq, bshp, bcps);
+        if (L.n_cand) rescore_candidates<THREADS, TILE_WORDS, SL>(a, L, P, g_cand, bkey, beq, bshp, bcps, n_errors);
         my_scanned += n_scanned;
         my_errors += n_errors;
         // ---- block reduction of the top hit; the spectrum's results ----

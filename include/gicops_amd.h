@@ -153,7 +153,7 @@ typedef struct gx_search_stats {
     uint64_t peaks;       /* query peaks read */
     uint64_t fallbacks;   /* learned mode: spectra routed to the plain tail fit */
     uint64_t ions_scanned;/* ion ids the search kernel streamed (self-check: equals `ions`) */
-    uint64_t errors;      /* internal consistency failures (ids outside their tile); always 0 */
+    uint64_t errors;      /* consistency failures: a candidate whose exact recount (pass 2) is below min_shp although the scatter (pass 1) counted min_shp; always 0 */
     uint64_t items;       /* batched: (spectrum, bucket, tile) work items */
     uint32_t chunks;      /* batched: planning chunks */
     int32_t config;       /* batched: launch configuration used */
