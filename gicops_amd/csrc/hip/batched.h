@@ -48,6 +48,7 @@ struct PlanArgs {
     uint64_t *pos_off;
     // outputs of plan_slices
     uint32_t *pos;           // chunk-local slice positions: [a, z) per slice
+    const uint32_t *plan_order;  // the order in which plan_slices visits each chunk's spectra
     uint32_t chunk_s0;       // first spectrum of the current chunk
     uint32_t chunk_s1;
     unsigned long long *stats;

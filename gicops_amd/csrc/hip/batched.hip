@@ -198,14 +198,15 @@ __global__ void __launch_bounds__(1024) scan_kernel(PlanArgs a) {
     }
 }
 
-// one block per (bucket, spectrum) of the chunk, bucket-major: the blocks in
-// flight read one bucket's bA and iA, which stay in L2
+// one block per (bucket, spectrum) of the chunk, bucket-major and by precursor
+// mass within a bucket: the blocks in flight read one bucket's bA and iA with
+// similar windows, which stay in L2
 __global__ void __launch_bounds__(kPlanThreads) plan_slices_kernel(PlanArgs a) {
     __shared__ unsigned long long s_st[2];
     const int tid = threadIdx.x, lane = tid & 63;
     const uint32_t nb = (uint32_t)a.n_buckets;
     const uint32_t nspec = a.chunk_s1 - a.chunk_s0;
-    const uint32_t s = a.chunk_s0 + blockIdx.x % nspec;
+    const uint32_t s = a.plan_order[a.chunk_s0 + blockIdx.x % nspec];
     const uint32_t b = blockIdx.x / nspec;
     const uint32_t pi = s * nb + b;
     const uint32_t tiles = a.pair_tiles[pi];

@@ -258,6 +258,7 @@ extern "C" int gx_queries_upload(gx_ctx *c, const gx_query_batch *q, gx_dqueries
     d->n_peaks = q->n_peaks;
     d->max_peaks = maxp;
     d->has_dm = q->dm != nullptr;
+    d->h_prec.assign(q->precurse, q->precurse + q->n_spectra);
     auto fail = [&](int st) {
         gx_queries_free(d);
         return st;
@@ -291,7 +292,7 @@ extern "C" void gx_queries_free(gx_dqueries *d) {
                   d->hist,      d->lq,         d->li,            d->n_legal,      d->pair_lo,  d->pair_hi,
                   d->pair_tiles, d->pair_pack, d->pair_pos_off, d->n_items,     d->item_off, d->n_pos,
                   d->pos_off,   d->cpsms,      d->counters,      d->best_key,     d->best_eq,  d->best_shp,
-                  d->pos,       d->order,           d->l_feat,       d->l_h1,     d->l_h2,
+                  d->pos,       d->order,      d->porder,           d->l_feat,       d->l_h1,     d->l_h2,
                   d->l_anchor,  d->l_tgt,      d->l_loge,     d->l_out,         d->l_valid,      d->l_sparse, d->l_route};
     for (void *p : ps)
         if (p) (void)hipFree(p);
@@ -391,7 +392,7 @@ int ensure_batched(gx_dqueries *q, const gx_dindex *idx) {
     q->nb = idx->n_buckets;
     if (dalloc(&q->lq, np) || dalloc(&q->li, np) || dalloc(&q->n_legal, ns) || dalloc(&q->pair_lo, npairs) ||
         dalloc(&q->pair_hi, npairs) || dalloc(&q->pair_tiles, npairs) || dalloc(&q->pair_pack, npairs) ||
-        dalloc(&q->pair_pos_off, npairs) || dalloc(&q->n_items, ns) || dalloc(&q->best_key, ns) || dalloc(&q->order, ns) ||
+        dalloc(&q->pair_pos_off, npairs) || dalloc(&q->n_items, ns) || dalloc(&q->best_key, ns) || dalloc(&q->order, ns) || dalloc(&q->porder, ns) ||
         dalloc(&q->best_eq, ns) || dalloc(&q->best_shp, ns) ||
         dalloc(&q->item_off, ns + 1) || dalloc(&q->n_pos, ns) || dalloc(&q->pos_off, ns + 1) ||
         dalloc(&q->cpsms, ns) || (!q->hist && dalloc(&q->hist, ns * GX_HIST_BINS)))
@@ -459,6 +460,7 @@ Batched setup_batched(gx_ctx *c, const gx_dindex *idx, gx_dqueries *q, const gx_
     pa.n_pos = q->n_pos;
     pa.item_off = q->item_off;
     pa.pos_off = q->pos_off;
+    pa.plan_order = q->porder;
     pa.stats = c->d_stats;
     B.n_blocks = std::max(1, c->n_cu * B.cfg.blocks_per_cu);
     B.learned = p->evalue_mode == GX_EVALUE_LEARNED;
@@ -628,7 +630,26 @@ int search_batched(gx_ctx *c, const gx_dindex *idx, gx_dqueries *q, const gx_sea
         }
         for (uint32_t s = s0; s < s1; s++) q->h_order[cnt[65 - cls(s)]++] = s;
     }
-    if (ns) GX_HIP(hipMemcpyAsync(q->order, q->h_order.data(), sizeof(uint32_t) * ns, hipMemcpyHostToDevice, sm));
+    // the plan visits each chunk's spectra by precursor mass (1 Da classes):
+    // neighbouring blocks then share windows, and their boundary searches
+    // share cache lines
+    q->h_porder.resize(ns);
+    for (size_t k = 0; k < n_chunks; k++) {
+        const uint32_t s0 = q->bounds[k], s1 = q->bounds[k + 1];
+        constexpr uint32_t kCls = 8192;
+        std::vector<uint32_t> cnt(kCls + 1, 0);
+        auto cls = [&](uint32_t s) {
+            const float m = q->h_prec[s];
+            return m > 0.0f ? std::min<uint32_t>((uint32_t)m, kCls - 1) : 0u;
+        };
+        for (uint32_t s = s0; s < s1; s++) cnt[cls(s) + 1]++;
+        for (uint32_t c = 0; c < kCls; c++) cnt[c + 1] += cnt[c];
+        for (uint32_t s = s0; s < s1; s++) q->h_porder[s0 + cnt[cls(s)]++] = s;
+    }
+    if (ns) {
+        GX_HIP(hipMemcpyAsync(q->order, q->h_order.data(), sizeof(uint32_t) * ns, hipMemcpyHostToDevice, sm));
+        GX_HIP(hipMemcpyAsync(q->porder, q->h_porder.data(), sizeof(uint32_t) * ns, hipMemcpyHostToDevice, sm));
+    }
     if (grow(&q->pos, q->pos_cap, max_chunk, q->buf_gen)) return GX_ERR_BAD_MEM_ALLOC;
     if (!q->counters || n_chunks > q->counters_n) {
         q->buf_gen++;

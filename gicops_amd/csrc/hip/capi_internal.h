@@ -83,8 +83,10 @@ struct gx_dqueries {
     size_t counters_n = 0;
     unsigned long long *best_key = nullptr;  // per-spectrum top hit of the search kernel
     uint32_t *best_eq = nullptr, *best_shp = nullptr;
-    uint32_t *order = nullptr;             // spectra of each chunk, largest first
-    std::vector<uint32_t> h_order;
+    uint32_t *order = nullptr;             // spectra of each chunk, largest first (search)
+    uint32_t *porder = nullptr;            // spectra of each chunk by precursor mass (plan)
+    std::vector<uint32_t> h_order, h_porder;
+    std::vector<float> h_prec;             // precursor masses (host copy)
     uint32_t *pos = nullptr;
     size_t pos_cap = 0;  // words
     std::vector<uint64_t> h_pos_off;
