@@ -198,6 +198,21 @@ __global__ void __launch_bounds__(1024) scan_kernel(PlanArgs a) {
     }
 }
 
+// first position p in [st, en) of a bin's ion ids with iA[p] >= x (en if
+// none): a search over the bin's samples iS (every 32nd id, a 32x smaller
+// array that stays in L2) brackets p within 32 ids, then a search of those
+__device__ __forceinline__ uint32_t lower_bound_sampled(const DevBucket &bk, uint32_t st, uint32_t en, uint32_t x) {
+    uint32_t lo = st, hi = en;
+    const uint32_t k0 = (st + 31u) >> 5;
+    if (en > 0 && k0 <= ((en - 1u) >> 5)) {
+        const uint32_t k1 = (en - 1u) >> 5;
+        const uint32_t kk = k0 + lower_bound_u32(gptr(bk.iS) + k0, k1 - k0 + 1u, x);
+        if (kk <= k1) hi = min(hi, kk << 5);                  // iA[32 kk] >= x
+        if (kk > k0) lo = max(lo, ((kk - 1u) << 5) + 1u);    // iA[32 (kk - 1)] < x
+    }
+    return lo + lower_bound_u32(gptr(bk.iA) + lo, hi - lo, x);
+}
+
 // one block per (bucket, spectrum) of the chunk, bucket-major and by precursor
 // mass within a bucket: the blocks in flight read one bucket's bA and iA with
 // similar windows, which stay in L2
@@ -243,7 +258,7 @@ __global__ void __launch_bounds__(kPlanThreads) plan_slices_kernel(PlanArgs a) {
         uint32_t cur = st;
         for (uint32_t t = 0; t <= tiles; t++) {
             const uint32_t id = (t == tiles) ? end_id : (lo + t * T) * speclen;
-            if (en > cur) cur += lower_bound_u32(gptr(bk.iA) + cur, en - cur, id);
+            if (en > cur) cur = lower_bound_sampled(bk, cur, en, id);
             out[t] = cur;
         }
         bins++;

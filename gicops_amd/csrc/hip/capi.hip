@@ -195,24 +195,28 @@ extern "C" int gx_index_upload(gx_ctx *c, const gx_bucket *b, int nb, int maxz, 
         if (x.n_entries > 0 && !make_divisor(x.speclen, db.div_m, db.div_s)) return fail(GX_ERR_INVLD_INDEX);
         if (x.n_entries > 0) {
             float *m;
-            uint32_t *bA, *iA, *ib;
+            uint32_t *bA, *iA, *ib, *is;
             if (dalloc(&m, x.n_entries) || dalloc(&bA, (size_t)x.nbins + 1) || dalloc(&iA, x.n_ions) ||
-                dalloc(&ib, x.n_ions)) {
+                dalloc(&ib, x.n_ions) || dalloc(&is, (x.n_ions + 31) / 32)) {
                 return fail(GX_ERR_BAD_MEM_ALLOC);
             }
             d->allocs.push_back(m);
             d->allocs.push_back(bA);
             d->allocs.push_back(iA);
             d->allocs.push_back(ib);
+            d->allocs.push_back(is);
             if (hipMemcpy(m, x.mass, sizeof(float) * x.n_entries, hipMemcpyHostToDevice) != hipSuccess ||
                 hipMemcpy(bA, x.bA, sizeof(uint32_t) * ((size_t)x.nbins + 1), hipMemcpyHostToDevice) != hipSuccess ||
                 hipMemcpy(iA, x.iA, sizeof(uint32_t) * x.n_ions, hipMemcpyHostToDevice) != hipSuccess)
                 return fail(GX_ERR_DEVICE);
-            if (launch_ion_bin(bA, iA, ib, x.nbins, x.n_ions, c->stream) != hipSuccess) return fail(GX_ERR_DEVICE);
+            if (launch_ion_bin(bA, iA, ib, x.nbins, x.n_ions, c->stream) != hipSuccess ||
+                launch_sample32(iA, is, x.n_ions, c->stream) != hipSuccess)
+                return fail(GX_ERR_DEVICE);
             db.mass = m;
             db.bA = bA;
             db.iA = iA;
             db.ion_bin = ib;
+            db.iS = is;
             d->max_speclen = std::max(d->max_speclen, x.speclen);
         }
         d->h_buckets[i] = db;
@@ -844,7 +848,7 @@ extern "C" int gx_autotune(gx_ctx *c, const gx_dindex *idx, gx_dqueries *q, cons
             if (r) return r;
         }
     const int cfg1 = bc, sp1 = bs;
-    for (int wb : {64, 128, 192, 256}) {
+    for (int wb : {32, 64, 128, 256}) {
         if (wb == w0) continue;
         double t;
         const int r = trial(cfg1, sp1, wb, t);

@@ -16,6 +16,7 @@ struct DevBucket {
     const uint32_t *bA;
     const uint32_t *iA;
     const uint32_t *ion_bin;
+    const uint32_t *iS;     // every 32nd ion id: iS[k] = iA[32k] (boundary searches)
     uint32_t n_entries;
     uint32_t speclen;
     // exact division by speclen for n < 2^31: n / speclen == umulhi(n, div_m) >> div_s
@@ -69,6 +70,7 @@ constexpr int kStatRows = 64;        // stat counters are spread over rows to av
 
 hipError_t launch_reset(uint32_t *counter, unsigned long long *stats, hipStream_t s);
 hipError_t launch_search(const SearchArgs &a, int n_blocks, hipStream_t s);
+hipError_t launch_sample32(const uint32_t *iA, uint32_t *iS, uint64_t n_ions, hipStream_t s);
 hipError_t launch_ion_bin(const uint32_t *bA, const uint32_t *iA, uint32_t *ion_bin, uint32_t nbins,
                           uint64_t n_ions, hipStream_t s);
 

@@ -75,7 +75,7 @@ def test_autotune_choice_keeps_parity(open_index, searcher_open):
     dq = searcher_open.upload(sample)
     r = searcher_open.autotune(dq, SearchParams(dM=500.0))
     dq.free()
-    assert r["config"] in (0, 1, 2) and r["split"] in (1, 2) and r["whole_bin"] in (64, 128, 192, 256)
+    assert r["config"] in (0, 1, 2) and r["split"] in (1, 2) and r["whole_bin"] in (32, 64, 128, 256)
     assert len(r["tried"]) == 9 and all(ms > 0 for _, _, _, ms in r["tried"])
     batch = _uneven(open_index, n=100, seed=46)
     g, hg, st = searcher_open.search(batch, SearchParams(dM=500.0), want_hist=True)  # tuned defaults
