@@ -9,13 +9,13 @@ namespace gx {
 // One search configuration (the autotuner's choice): block size and the width
 // of the LDS count tile. Counts are packed 4 (u8) or 2 (u16) per 32-bit word.
 struct BatchConfig {
-    int threads;     // 256, 512 or 1024
+    int threads;     // 256, 512, 768 or 1024
     int tile_words;  // count-tile words per block
     int slice_chunk; // slices staged in LDS at a time
     int blocks_per_cu;
 };
 
-constexpr int kNumBatchConfigs = 3;
+constexpr int kNumBatchConfigs = 4;
 BatchConfig batch_config(int id);
 
 struct PlanArgs {

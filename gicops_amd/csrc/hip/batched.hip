@@ -757,7 +757,8 @@ BatchConfig batch_config(int id) {
     switch (id) {
         case 0: return {1024, 28672, 1024, 1};  // 112 KB tile, one block per CU
         case 1: return {512, 12288, 512, 2};    // 48 KB tile, two blocks per CU
-        default: return {256, 4096, 256, 3};    // 16 KB tile, three blocks per CU
+        case 2: return {256, 4096, 256, 3};     // 16 KB tile, three blocks per CU
+        default: return {768, 28672, 768, 1};   // 112 KB tile, one block of 12 waves per CU
     }
 }
 
@@ -791,7 +792,8 @@ hipError_t launch_search_spectra(int config, const ItemArgs &a, int n_blocks, hi
     switch (config) {
         case 0: return launch_spectra<1024, 28672, 1024, 4>(a, n_blocks, s);
         case 1: return launch_spectra<512, 12288, 512, 4>(a, n_blocks, s);
-        default: return launch_spectra<256, 4096, 256, 3>(a, n_blocks, s);
+        case 2: return launch_spectra<256, 4096, 256, 3>(a, n_blocks, s);
+        default: return launch_spectra<768, 28672, 768, 3>(a, n_blocks, s);
     }
 }
 

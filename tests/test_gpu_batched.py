@@ -75,8 +75,8 @@ def test_autotune_choice_keeps_parity(open_index, searcher_open):
     dq = searcher_open.upload(sample)
     r = searcher_open.autotune(dq, SearchParams(dM=500.0))
     dq.free()
-    assert r["config"] in (0, 1, 2) and r["split"] in (1, 2) and r["whole_bin"] in (32, 64, 128, 256)
-    assert len(r["tried"]) == 9 and all(ms > 0 for _, _, _, ms in r["tried"])
+    assert r["config"] in (0, 1, 2, 3) and r["split"] in (1, 2) and r["whole_bin"] in (32, 64, 128, 256)
+    assert len(r["tried"]) == 11 and all(ms > 0 for _, _, _, ms in r["tried"])
     batch = _uneven(open_index, n=100, seed=46)
     g, hg, st = searcher_open.search(batch, SearchParams(dM=500.0), want_hist=True)  # tuned defaults
     assert st["config"] == r["config"]

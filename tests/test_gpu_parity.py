@@ -45,8 +45,9 @@ def compare(gpu, cpu, hg=None, hc=None, label=""):
 
 # every search engine and batched launch configuration
 ENGINES = [dict(engine=F.GX_ENGINE_BATCHED, config=0), dict(engine=F.GX_ENGINE_BATCHED, config=1),
-           dict(engine=F.GX_ENGINE_BATCHED, config=2), dict(engine=F.GX_ENGINE_SPECTRUM)]
-ENGINE_IDS = ["batched-c0", "batched-c1", "batched-c2", "spectrum"]
+           dict(engine=F.GX_ENGINE_BATCHED, config=2), dict(engine=F.GX_ENGINE_BATCHED, config=3),
+           dict(engine=F.GX_ENGINE_SPECTRUM)]
+ENGINE_IDS = ["batched-c0", "batched-c1", "batched-c2", "batched-c3", "spectrum"]
 
 
 def check_stats(st, cnt, n):
