@@ -331,7 +331,7 @@ __device__ void rescore_candidates(const ItemArgs &a, SpecLds<THREADS, TILE_WORD
     for (uint32_t c = tid; c < ncell; c += THREADS) first[c] = (uint16_t)lower_bound_lds(L.pk_q, P, (int32_t)(c << cs));
     uint32_t *acc_n = L.cnt, *acc_ib = L.cnt + CB, *acc_iy = L.cnt + 2 * CB;
     const uint32_t smax = a.max_speclen;
-    __syncthreads();
+    lds_barrier();
     for (uint32_t c0 = 0; c0 < ncand; c0 += CB) {
         const uint32_t nc = min((uint32_t)CB, ncand - c0);
         const uint32_t tasks = nc * smax;
@@ -372,7 +372,7 @@ __device__ void rescore_candidates(const ItemArgs &a, SpecLds<THREADS, TILE_WORD
                 }
             }
         }
-        __syncthreads();
+        lds_barrier();
         for (uint32_t c = tid; c < nc; c += THREADS) {
             const uint32_t v = acc_n[c];
             const uint32_t bc = v & 0xffffu, yc = v >> 16;
@@ -396,11 +396,11 @@ __device__ void rescore_candidates(const ItemArgs &a, SpecLds<THREADS, TILE_WORD
                 }
             }
         }
-        __syncthreads();
+        lds_barrier();
     }
     for (uint32_t c = tid; c < (ncell + 1) / 2; c += THREADS) L.cnt[TILE_WORDS - kPeakIdxWords + c] = 0;
     if (tid == 0) L.n_cand = 0;
-    __syncthreads();
+    lds_barrier();
 }
 
 // Persistent search: blocks pull whole spectra of the chunk from a counter.
@@ -432,11 +432,11 @@ __global__ void __launch_bounds__(THREADS, WAVES_PER_SIMD) search_spectra_kernel
     unsigned long long my_cand = 0, my_scanned = 0, my_errors = 0;
 
     for (;;) {
-        __syncthreads();
+        lds_barrier();
         const uint32_t ticket = L.next;
         if (ticket >= nspec) break;
         const uint32_t s = a.order[a.chunk_s0 + ticket];
-        __syncthreads();
+        lds_barrier();
         // fetch the next ticket now; its latency overlaps this spectrum
         if (tid == 0) L.next = atomicAdd(a.work_counter, 1u);
         const int P = (int)a.n_legal[s];
@@ -486,7 +486,8 @@ __global__ void __launch_bounds__(THREADS, WAVES_PER_SIMD) search_spectra_kernel
                 const uint32_t nent = (uint32_t)(thi - tlo + 1);
                 const uint32_t tlo_id = (uint32_t)tlo * speclen;
                 const uint32_t span_id = nent * speclen;
-                // room for every entry of this tile as a candidate
+                // room for every entry of this tile as a candidate (a full barrier:
+                // pass 2 reads candidates that pass 1 wrote to global scratch)
                 __syncthreads();
                 if (L.n_cand + nent > cand_cap)
                     rescore_candidates<THREADS, TILE_WORDS, SL>(a, L, P, g_cand, bkey, beq, bshp, bcps, n_errors);
@@ -515,7 +516,7 @@ __global__ void __launch_bounds__(THREADS, WAVES_PER_SIMD) search_spectra_kernel
                         L.wsum[wave] = il;
                         L.wsum2[wave] = ifl;
                     }
-                    __syncthreads();
+                    lds_barrier();
                     uint32_t bl = 0, bf = 0;
                     for (int w = 0; w < wave; w++) {
                         bl += L.wsum[w];
@@ -536,7 +537,7 @@ __global__ void __launch_bounds__(THREADS, WAVES_PER_SIMD) search_spectra_kernel
                         L.s_flat[bf + ifl] = bl + il;
                         L.ne = bf + ifl;
                     }
-                    __syncthreads();
+                    lds_barrier();
                     const uint32_t ne = L.ne;
                     const uint32_t N = L.s_flat[ne];
                     const uint32_t nch = (N + 63) >> 6;
@@ -544,19 +545,19 @@ __global__ void __launch_bounds__(THREADS, WAVES_PER_SIMD) search_spectra_kernel
                     for (uint32_t r0 = 0; r0 < nch; r0 += THREADS) {
                         const uint32_t nr = min((uint32_t)THREADS, nch - r0);
                         if (r0 > 0) {  // later rounds: the masks were cleared after the previous one
-                            __syncthreads();
+                            lds_barrier();
                             if ((uint32_t)tid < ne) {
                                 const uint32_t f = L.s_flat[tid];
                                 const uint32_t c = (f >> 6) - r0;
                                 if (f >= (r0 << 6) && c < nr) atomicOr(&L.cmask[c], 1ull << (f & 63u));
                             }
-                            __syncthreads();
+                            lds_barrier();
                         }
                         {
                             const uint32_t v = (uint32_t)tid < nr ? (uint32_t)__popcll(L.cmask[tid]) : 0u;
                             const uint32_t inc = wave_incl_scan(v, lane);
                             if (lane == 63) L.wsum[wave] = inc;
-                            __syncthreads();
+                            lds_barrier();
                             uint32_t wb = 0, tot = 0;
                             for (int w = 0; w < NW; w++) {
                                 const uint32_t x = L.wsum[w];
@@ -566,7 +567,7 @@ __global__ void __launch_bounds__(THREADS, WAVES_PER_SIMD) search_spectra_kernel
                             if ((uint32_t)tid < nr) L.cj0[tid] = (uint16_t)(base + wb + inc - v);
                             base += tot;
                         }
-                        __syncthreads();
+                        lds_barrier();
                         // U chunks per wave in three batched LDS/HBM stages: chunk
                         // metadata, slice bases, ion ids; lane L's slice is the chunk's
                         // first plus the slice starts at lanes <= L
@@ -619,11 +620,11 @@ __global__ void __launch_bounds__(THREADS, WAVES_PER_SIMD) search_spectra_kernel
                                 }
                             }
                         }
-                        __syncthreads();
+                        lds_barrier();
                         L.cmask[tid] = 0;  // ordered before the next use by at least one barrier
                     }
                 }
-                __syncthreads();
+                lds_barrier();
                 // ---- the candidates were collected in pass 1; clear the tile ----
                 {
                     const uint32_t nq = (((nent + pack - 1) >> lp) + 3) >> 2;
@@ -632,7 +633,7 @@ __global__ void __launch_bounds__(THREADS, WAVES_PER_SIMD) search_spectra_kernel
                 }
             }
         }
-        __syncthreads();
+        __syncthreads();  // candidates in global scratch, see above
         if (L.n_cand) rescore_candidates<THREADS, TILE_WORDS, SL>(a, L, P, g_cand, bkey, beq, bshp, bcps, n_errors);
         my_scanned += n_scanned;
         my_errors += n_errors;
@@ -650,7 +651,7 @@ __global__ void __launch_bounds__(THREADS, WAVES_PER_SIMD) search_spectra_kernel
             L.w_shp[wave] = bshp;
             L.w_cps[wave] = bcps;
         }
-        __syncthreads();
+        lds_barrier();
         for (int w = tid; w < GX_HIST_BINS; w += THREADS) {
             a.hist[(size_t)s * GX_HIST_BINS + w] = L.hist[w];
             L.hist[w] = 0;

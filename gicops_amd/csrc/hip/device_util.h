@@ -23,6 +23,16 @@ __device__ __forceinline__ gmem<T> *gptr(const T *p) {
     return (gmem<T> *)p;
 }
 
+// Block barrier that orders LDS only. __syncthreads() also waits for every
+// outstanding global load (its release fence covers global memory), which
+// would stall loads issued ahead of time; this one leaves them in flight.
+// Use it only where no global data written by one thread is read by another.
+__device__ __forceinline__ void lds_barrier() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+    __builtin_amdgcn_s_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
 template <class P>
 __device__ __forceinline__ uint32_t lower_bound_u32(P a, uint32_t n, uint32_t v) {
     uint32_t lo = 0;
