@@ -36,6 +36,7 @@ HIP_SOURCES = [
     CSRC / "hip" / "capi.hip",
     CSRC / "hip" / "mlp.hip",
     CSRC / "hip" / "mlp_capi.hip",
+    CSRC / "hip" / "index_build.hip",
 ]
 HIP_HEADERS = sorted((CSRC / "hip").glob("*.h")) + sorted(INCLUDE.glob("*.h"))
 

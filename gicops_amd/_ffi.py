@@ -137,6 +137,16 @@ class gx_db_params(C.Structure):
     ]
 
 
+GX_MAX_MOD_TYPES = 15
+
+
+class gx_index_build_params(C.Structure):
+    _fields_ = [
+        ("maxz", C.c_int32), ("scale", C.c_int32), ("min_mass", C.c_float), ("max_mass", C.c_float),
+        ("n_mod_types", C.c_int32), ("mod_mass", C.c_uint32 * GX_MAX_MOD_TYPES), ("modmass_all", C.c_int32),
+    ]
+
+
 class gx_spec_params(C.Structure):
     _fields_ = [
         ("n_spectra", C.c_uint32), ("keep_frac", C.c_float), ("mz_sigma", C.c_float),
@@ -216,6 +226,8 @@ def host_lib() -> C.CDLL:
         lib.gx_db_add_peptides.argtypes = [vp, C.c_char_p, vp, u64, C.POINTER(u64)]
         lib.gx_db_generate_peptides.argtypes = [vp, u64, u64]
         lib.gx_db_finalize.argtypes = [vp]
+        lib.gx_db_finalize_entries.argtypes = [vp]
+        lib.gx_db_build_params.argtypes = [vp, C.POINTER(gx_index_build_params)]
         lib.gx_db_num_buckets.argtypes = [vp]
         lib.gx_db_get_bucket.argtypes = [vp, i32, C.POINTER(gx_bucket)]
         lib.gx_db_destroy.argtypes = [vp]
@@ -238,6 +250,9 @@ def hip_lib() -> C.CDLL:
         lib.gx_index_upload.argtypes = [vp, C.POINTER(gx_bucket), C.c_int, C.c_int, C.POINTER(vp)]
         lib.gx_index_free.argtypes = [vp]
         lib.gx_index_free.restype = None
+        lib.gx_index_build.argtypes = [vp, C.POINTER(gx_bucket), C.c_int, C.POINTER(gx_index_build_params),
+                                       C.POINTER(vp)]
+        lib.gx_index_download.argtypes = [vp, C.c_int, vp, vp]
         lib.gx_queries_upload.argtypes = [vp, C.POINTER(gx_query_batch), C.POINTER(vp)]
         lib.gx_queries_free.argtypes = [vp]
         lib.gx_queries_free.restype = None

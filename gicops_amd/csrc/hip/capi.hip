@@ -160,6 +160,16 @@ extern "C" int gx_index_upload(gx_ctx *c, const gx_bucket *b, int nb, int maxz, 
     *out = nullptr;
     if (nb < 1 || nb > kMaxBuckets || maxz < 1) return GX_ERR_INVLD_PARAM;
     std::lock_guard<std::mutex> lk(c->mu);
+    return gx::index_create(c, b, nb, maxz, nullptr, nullptr, out);
+}
+
+namespace gx {
+
+// A device index from bucket metadata. bA/iA come from the host buckets, or,
+// when dev_bA/dev_iA are given, are device arrays the index takes over.
+int index_create(gx_ctx *c, const gx_bucket *b, int nb, int maxz, uint32_t *const *dev_bA, uint32_t *const *dev_iA,
+                 gx_dindex **out) {
+    const bool on_dev = dev_bA && dev_iA;
     if (hipSetDevice(c->device) != hipSuccess) return GX_ERR_DEVICE;
     gx_dindex *d = new (std::nothrow) gx_dindex();
     if (!d) return GX_ERR_BAD_MEM_ALLOC;
@@ -167,6 +177,11 @@ extern "C" int gx_index_upload(gx_ctx *c, const gx_bucket *b, int nb, int maxz, 
     d->n_buckets = nb;
     d->maxz = maxz;
     d->h_buckets.resize(nb);
+    if (on_dev)  // the index owns the given arrays from here on, also on failure
+        for (int i = 0; i < nb; i++) {
+            if (dev_bA[i]) d->allocs.push_back(dev_bA[i]);
+            if (dev_iA[i]) d->allocs.push_back(dev_iA[i]);
+        }
     auto fail = [&](int st) {
         for (void *p : d->allocs) (void)hipFree(p);
         if (d->d_buckets) (void)hipFree(d->d_buckets);
@@ -177,12 +192,12 @@ extern "C" int gx_index_upload(gx_ctx *c, const gx_bucket *b, int nb, int maxz, 
     for (int i = 0; i < nb; i++) {
         const gx_bucket &x = b[i];
         if (x.n_entries == 0) continue;
-        if (!x.mass || !x.bA || !x.iA) return fail(GX_ERR_INVLD_PTR);
+        if (!x.mass || (!on_dev && (!x.bA || !x.iA))) return fail(GX_ERR_INVLD_PTR);
         if (nbins == 0) nbins = x.nbins;
         if (x.nbins != nbins || x.nbins < 2) return fail(GX_ERR_INVLD_INDEX);
         if (x.speclen != (uint32_t)((x.peplen - 1) * maxz * 2) || x.speclen < 2) return fail(GX_ERR_INVLD_INDEX);
         if ((uint64_t)x.n_entries * x.speclen != x.n_ions || x.n_ions >= (1ull << 32)) return fail(GX_ERR_INVLD_SIZE);
-        if (x.bA[x.nbins] != x.n_ions) return fail(GX_ERR_INVLD_INDEX);
+        if (!on_dev && x.bA[x.nbins] != x.n_ions) return fail(GX_ERR_INVLD_INDEX);
         if (x.n_entries >= (1u << 26)) return fail(GX_ERR_INVLD_SIZE);  // packed top-hit key holds 26-bit psids
         if (x.n_ions >= (1ull << 31)) return fail(GX_ERR_INVLD_SIZE);     // ion ids use the 31-bit divisor
     }
@@ -194,20 +209,30 @@ extern "C" int gx_index_upload(gx_ctx *c, const gx_bucket *b, int nb, int maxz, 
         db.speclen = x.speclen;
         if (x.n_entries > 0 && !make_divisor(x.speclen, db.div_m, db.div_s)) return fail(GX_ERR_INVLD_INDEX);
         if (x.n_entries > 0) {
-            float *m;
-            uint32_t *bA, *iA, *ib, *is;
-            if (dalloc(&m, x.n_entries) || dalloc(&bA, (size_t)x.nbins + 1) || dalloc(&iA, x.n_ions) ||
+            float *m = nullptr;
+            uint32_t *bA = on_dev ? dev_bA[i] : nullptr, *iA = on_dev ? dev_iA[i] : nullptr, *ib = nullptr,
+                     *is = nullptr;
+            if (on_dev && (!bA || !iA)) return fail(GX_ERR_INVLD_PTR);
+            if (dalloc(&m, x.n_entries) || (!on_dev && (dalloc(&bA, (size_t)x.nbins + 1) || dalloc(&iA, x.n_ions))) ||
                 dalloc(&ib, x.n_ions) || dalloc(&is, (x.n_ions + 31) / 32)) {
+                for (void *q : {(void *)m, (void *)ib, (void *)is}) if (q) (void)hipFree(q);
+                if (!on_dev) {
+                    if (bA) (void)hipFree(bA);
+                    if (iA) (void)hipFree(iA);
+                }
                 return fail(GX_ERR_BAD_MEM_ALLOC);
             }
             d->allocs.push_back(m);
-            d->allocs.push_back(bA);
-            d->allocs.push_back(iA);
+            if (!on_dev) {
+                d->allocs.push_back(bA);
+                d->allocs.push_back(iA);
+            }
             d->allocs.push_back(ib);
             d->allocs.push_back(is);
             if (hipMemcpy(m, x.mass, sizeof(float) * x.n_entries, hipMemcpyHostToDevice) != hipSuccess ||
-                hipMemcpy(bA, x.bA, sizeof(uint32_t) * ((size_t)x.nbins + 1), hipMemcpyHostToDevice) != hipSuccess ||
-                hipMemcpy(iA, x.iA, sizeof(uint32_t) * x.n_ions, hipMemcpyHostToDevice) != hipSuccess)
+                (!on_dev &&
+                 (hipMemcpy(bA, x.bA, sizeof(uint32_t) * ((size_t)x.nbins + 1), hipMemcpyHostToDevice) != hipSuccess ||
+                  hipMemcpy(iA, x.iA, sizeof(uint32_t) * x.n_ions, hipMemcpyHostToDevice) != hipSuccess)))
                 return fail(GX_ERR_DEVICE);
             if (launch_ion_bin(bA, iA, ib, x.nbins, x.n_ions, c->stream) != hipSuccess ||
                 launch_sample32(iA, is, x.n_ions, c->stream) != hipSuccess)
@@ -230,6 +255,8 @@ extern "C" int gx_index_upload(gx_ctx *c, const gx_bucket *b, int nb, int maxz, 
     *out = d;
     return GX_SUCCESS;
 }
+
+}  // namespace gx
 
 extern "C" void gx_index_free(gx_dindex *d) {
     if (!d) return;

@@ -122,6 +122,11 @@ int param_error(int code, const char *what);
     } while (0)
 
 namespace gx {
+// a device index from bucket metadata; bA/iA from the host buckets, or device
+// arrays the index takes over (capi.hip)
+int index_create(gx_ctx *c, const gx_bucket *b, int nb, int maxz, uint32_t *const *dev_bA, uint32_t *const *dev_iA,
+                 gx_dindex **out);
+
 // learned e-value stage of a batched search (mlp_capi.hip)
 int ensure_learned(gx_ctx *c, gx_dqueries *q);
 hipError_t enqueue_learned_evalue(gx_ctx *c, gx_dqueries *q, EvalArgs ea, float conf, hipStream_t s);

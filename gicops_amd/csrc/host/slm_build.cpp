@@ -219,6 +219,7 @@ struct gx_db {
     gx::ModTable mt;
     std::vector<gx::Bucket> buckets;
     bool finalized = false;
+    bool has_fragments = false;  // bA/iA built on the host
 };
 
 using namespace gx;
@@ -293,7 +294,7 @@ extern "C" int gx_db_generate_peptides(gx_db *db, uint64_t seed, uint64_t n) {
     return GX_SUCCESS;
 }
 
-static int build_bucket(gx_db *db, Bucket &b) {
+static int build_bucket(gx_db *db, Bucket &b, bool fragments) {
     const gx_db_params &p = db->p;
     const int L = b.peplen;
     const uint32_t npep = b.n_peps();
@@ -336,6 +337,7 @@ static int build_bucket(gx_db *db, Bucket &b) {
         b.sites[k] = b.entries[k].sites;
     }
     std::vector<Entry>().swap(b.entries);
+    if (!fragments) return GX_SUCCESS;  // bA/iA are built elsewhere (gx_index_build)
     // 3. fragment bins of every entry
     const uint32_t nbins = (uint32_t)(p.max_mass * p.scale);
     const uint64_t nions = nent * speclen;
@@ -380,7 +382,7 @@ static int build_bucket(gx_db *db, Bucket &b) {
     return GX_SUCCESS;
 }
 
-extern "C" int gx_db_finalize(gx_db *db) {
+static int finalize(gx_db *db, bool fragments) {
     if (!db) return GX_ERR_INVLD_PTR;
     if (db->finalized) return GX_ERR_ALREADY_INIT;
     const int nthr = db->p.threads > 0 ? db->p.threads : omp_get_max_threads();
@@ -389,15 +391,35 @@ extern "C" int gx_db_finalize(gx_db *db) {
     int st = GX_SUCCESS;
     try {
         for (auto &b : db->buckets) {
-            st = build_bucket(db, b);
+            st = build_bucket(db, b, fragments);
             if (st != GX_SUCCESS) break;
         }
     } catch (const std::bad_alloc &) {
         st = GX_ERR_BAD_MEM_ALLOC;
     }
     omp_set_num_threads(saved);
-    if (st == GX_SUCCESS) db->finalized = true;
+    if (st == GX_SUCCESS) {
+        db->finalized = true;
+        db->has_fragments = fragments;
+    }
     return st;
+}
+
+extern "C" int gx_db_finalize(gx_db *db) { return finalize(db, true); }
+
+extern "C" int gx_db_finalize_entries(gx_db *db) { return finalize(db, false); }
+
+extern "C" int gx_db_build_params(const gx_db *db, gx_index_build_params *out) {
+    if (!db || !out) return GX_ERR_INVLD_PTR;
+    std::memset(out, 0, sizeof(*out));
+    out->maxz = db->p.maxz;
+    out->scale = db->p.scale;
+    out->min_mass = db->p.min_mass;
+    out->max_mass = db->p.max_mass;
+    out->n_mod_types = db->mt.n_types;
+    for (int t = 0; t < db->mt.n_types; t++) out->mod_mass[t] = db->mt.mod_mass[t];
+    out->modmass_all = db->p.modmass_all;
+    return GX_SUCCESS;
 }
 
 extern "C" int gx_db_num_buckets(const gx_db *db) { return db ? (int)db->buckets.size() : 0; }
@@ -411,15 +433,15 @@ extern "C" int gx_db_get_bucket(const gx_db *db, int i, gx_bucket *out) {
     out->n_peps = b.n_peps();
     out->n_entries = (uint32_t)b.mass.size();
     out->speclen = (uint32_t)((b.peplen - 1) * db->p.maxz * 2);
-    out->n_ions = b.iA.size();
+    out->n_ions = (uint64_t)b.mass.size() * out->speclen;
     out->nbins = (uint32_t)(db->p.max_mass * db->p.scale);
     out->seqs = b.seqs.data();
     out->mass = b.mass.data();
     out->seqid = b.seqid.data();
     out->modnum = b.modnum.data();
     out->sites = b.sites.data();
-    out->bA = b.bA.data();
-    out->iA = b.iA.data();
+    out->bA = db->has_fragments ? b.bA.data() : nullptr;
+    out->iA = db->has_fragments ? b.iA.data() : nullptr;
     return GX_SUCCESS;
 }
 

@@ -78,15 +78,33 @@ class DeviceQueries:
 
 
 class Searcher:
-    def __init__(self, index: SlmIndex, device: int = 0):
+    def __init__(self, index: SlmIndex, device: int = 0, build_on_device: bool | None = None):
+        """Uploads the host-built fragment-ion index, or builds it on the GPU
+        from the index's mass-sorted entries (gx_index_build) when
+        `build_on_device` is set; None builds on the GPU exactly when the host
+        index carries no fragment arrays."""
         self._lib = F.hip_lib()
         ctx = C.c_void_p()
         F.check("gx_ctx_create", self._lib.gx_ctx_create(device, C.byref(ctx)))
         self._ctx = ctx
         self.index = index
+        if build_on_device is None:
+            build_on_device = not index.has_fragments
+        if not build_on_device and not index.has_fragments:
+            raise ValueError("the index has no host fragment arrays to upload; build it on the device")
+        self.built_on_device = build_on_device
         d = C.c_void_p()
-        F.check("gx_index_upload", self._lib.gx_index_upload(ctx, index.raw_array, len(index.buckets),
-                                                              index.maxz, C.byref(d)))
+        if build_on_device:
+            bp = index.build_params()
+            st = self._lib.gx_index_build(ctx, index.raw_array, len(index.buckets), C.byref(bp), C.byref(d))
+            name = "gx_index_build"
+        else:
+            st = self._lib.gx_index_upload(ctx, index.raw_array, len(index.buckets), index.maxz, C.byref(d))
+            name = "gx_index_upload"
+        if st:
+            self._lib.gx_ctx_destroy(ctx)
+            self._ctx = None
+            F.check(name, st)
         self._didx = d
         self._children: list = []  # surrogates, closed before the context
 
@@ -95,6 +113,16 @@ class Searcher:
         ncu = C.c_int()
         F.check("gx_ctx_device_info", self._lib.gx_ctx_device_info(self._ctx, buf, 256, C.byref(ncu)))
         return buf.value.decode(), ncu.value
+
+    def index_arrays(self, bucket: int) -> tuple[np.ndarray, np.ndarray]:
+        """(bA, iA) of one bucket as the device index holds them."""
+        b = self.index.buckets[bucket]
+        bA = np.zeros(b.nbins + 1 if b.n_entries else 0, dtype=np.uint32)
+        iA = np.zeros(b.n_ions, dtype=np.uint32)
+        F.check("gx_index_download",
+                self._lib.gx_index_download(self._didx, bucket, F.ptr(bA) if bA.size else None,
+                                            F.ptr(iA) if iA.size else None))
+        return bA, iA
 
     def upload(self, batch: QueryBatch, with_dm: bool = True) -> DeviceQueries:
         return DeviceQueries(self, batch, with_dm)

@@ -112,7 +112,10 @@ class Bucket:
 class SlmIndex:
     """A finalized synthetic (or user-supplied) peptide index."""
 
-    def __init__(self, cfg: DBConfig, peptides: list[str] | None = None):
+    def __init__(self, cfg: DBConfig, peptides: list[str] | None = None, fragments: bool = True):
+        """`fragments=False` stops after the mass-sorted entries (no bA/iA on
+        the host): the fragment-ion arrays are then built on the GPU when a
+        Searcher takes the index (gx_index_build)."""
         lib = F.host_lib()
         p = F.gx_db_params()
         p.min_len, p.max_len, p.maxz, p.scale = cfg.min_len, cfg.max_len, cfg.maxz, cfg.scale
@@ -140,13 +143,24 @@ class SlmIndex:
         else:
             F.check("gx_db_generate_peptides", lib.gx_db_generate_peptides(h, cfg.seed, cfg.n_peptides))
             self.n_kept = cfg.n_peptides
-        F.check("gx_db_finalize", lib.gx_db_finalize(h))
+        self.has_fragments = fragments
+        if fragments:
+            F.check("gx_db_finalize", lib.gx_db_finalize(h))
+        else:
+            F.check("gx_db_finalize_entries", lib.gx_db_finalize_entries(h))
         self.buckets: list[Bucket] = []
         for i in range(lib.gx_db_num_buckets(h)):
             b = F.gx_bucket()
             F.check("gx_db_get_bucket", lib.gx_db_get_bucket(h, i, C.byref(b)))
             self.buckets.append(Bucket(b, self))
         self.raw_array = (F.gx_bucket * len(self.buckets))(*[b.raw for b in self.buckets])
+
+    def build_params(self) -> F.gx_index_build_params:
+        """The parameters gx_index_build needs to build this index's fragment
+        arrays (charges, scale, mass range, modification masses)."""
+        bp = F.gx_index_build_params()
+        F.check("gx_db_build_params", self._lib.gx_db_build_params(self._h, C.byref(bp)))
+        return bp
 
     @property
     def n_entries(self) -> int:

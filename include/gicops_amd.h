@@ -178,6 +178,30 @@ void gx_ctx_destroy(gx_ctx *ctx);
 int gx_index_upload(gx_ctx *ctx, const gx_bucket *buckets, int n_buckets, int maxz, gx_dindex **out);
 void gx_index_free(gx_dindex *idx);
 
+/* Build the fragment-ion index of every bucket on the GPU, replacing
+ * s1::ConstructIndexChunk (superstep1/kernel.cu:484-733) and the CPU's
+ * DSLIM_ConstructChunk + DSLIM_SLMTransform (dslim.cpp:89-97). `buckets`
+ * are the mass-sorted entries (mass, seqid, modnum, sites, seqs); their bA
+ * and iA are ignored (may be NULL). Fragment bins follow the CPU build's
+ * float arithmetic (utils.cpp:161-474, dslim.cpp:318-337), and iA holds ion
+ * ids ascending within each bin, so the result is bit-identical to the CPU
+ * index. The device index is ready for search, like gx_index_upload's. */
+#define GX_MAX_MOD_TYPES 15
+typedef struct gx_index_build_params {
+    int32_t maxz;
+    int32_t scale;
+    float min_mass;
+    float max_mass;
+    int32_t n_mod_types;
+    uint32_t mod_mass[GX_MAX_MOD_TYPES]; /* (uint)(mass * scale) per type (modMass, mods.cpp) */
+    int32_t modmass_all;                 /* 0: the reference's first-modification precursor quirk */
+} gx_index_build_params;
+int gx_index_build(gx_ctx *ctx, const gx_bucket *buckets, int n_buckets, const gx_index_build_params *bp,
+                   gx_dindex **out);
+/* Copy bucket `bucket`'s bA (nbins+1) and iA (n_ions) of a device index to
+ * the host (either pointer may be NULL). */
+int gx_index_download(gx_dindex *idx, int bucket, uint32_t *bA, uint32_t *iA);
+
 /* Upload a query batch to HBM (dQueries::H2D, superstep3/kernel.cu:192-206). */
 int gx_queries_upload(gx_ctx *ctx, const gx_query_batch *q, gx_dqueries **out);
 void gx_queries_free(gx_dqueries *q);

@@ -80,6 +80,13 @@ int gx_db_generate_peptides(gx_db *db, uint64_t seed, uint64_t n);
 /* Generate the variable-modification variants, sort every bucket by mass and
  * build bA/iA. Must be called once, after all peptides were added. */
 int gx_db_finalize(gx_db *db);
+/* Like gx_db_finalize, but without the fragment index: entries are built and
+ * sorted by mass, and gx_db_get_bucket returns bA = iA = NULL. The fragment
+ * index is then built on the GPU by gx_index_build (gicops_amd.h). */
+int gx_db_finalize_entries(gx_db *db);
+/* The fragment-build parameters of the database (charges, scale, mass
+ * range, modification masses) for gx_index_build. */
+int gx_db_build_params(const gx_db *db, gx_index_build_params *out);
 
 int gx_db_num_buckets(const gx_db *db);
 int gx_db_get_bucket(const gx_db *db, int i, gx_bucket *out);

@@ -12,6 +12,8 @@ index (2M tryptic peptides + Met oxidation, ~5.6M entries, ~290M ions):
   bit-identical to the eager run, the work counters equal the oracle's on the
   sample, and the e-value is cpsms * 10^(mu/1e6 * hyp + beta/1e6).
 """
+import time
+
 import numpy as np
 import pytest
 
@@ -34,7 +36,10 @@ def say(msg):
 @pytest.fixture(scope="module")
 def big_index():
     say("building the 2M-peptide open index")
-    return slm.SlmIndex(slm.DB_PRESETS["open2m"])
+    t0 = time.perf_counter()
+    index = slm.SlmIndex(slm.DB_PRESETS["open2m"])
+    say(f"  host build: {(time.perf_counter() - t0) * 1e3:.0f} ms")
+    return index
 
 
 @pytest.fixture(scope="module")
@@ -86,3 +91,19 @@ def test_full_size_workload(big_index, big_searcher, workload):
     assert gst["ions"] == int(cnt["ions"].sum()) and gst["window"] == int(cnt["window"].sum())
     assert gs.tobytes() == psm[idx].tobytes()
     dq.free()
+
+
+def test_full_size_device_build(big_index):
+    """The 2M-peptide open index built on the GPU equals the host build."""
+    say("case: full-size open2m index built on the GPU")
+    t0 = time.perf_counter()
+    gs = Searcher(big_index, build_on_device=True)
+    say(f"  device build (entries upload, fragments, sort, bin starts, derived arrays): "
+        f"{(time.perf_counter() - t0) * 1e3:.0f} ms for {big_index.n_ions} ions")
+    for i, b in enumerate(big_index.buckets):
+        if b.n_entries == 0:
+            continue
+        bA, iA = gs.index_arrays(i)
+        assert np.array_equal(bA, b.bA), f"bucket {i}: bA differs"
+        assert np.array_equal(iA, b.iA), f"bucket {i}: iA differs"
+    gs.close()

@@ -160,3 +160,24 @@ def test_hyperscore_closed_form(tiny_index):
             + math.log10(1 + int(ibc[0])) + math.log10(1 + int(iyc[0])) - 4
         assert abs(min(h, 79.0) - float(r["hyperscore"])) < 1e-5 * max(1.0, h)
         assert int(bc[0] + yc[0]) == int(r["sharedions"])
+
+
+def test_entries_only_index_and_build_params():
+    """gx_db_finalize_entries gives the same mass-sorted entries as the full
+    build, without host fragment arrays; gx_db_build_params reports the
+    build's charges, scale, mass range and modification masses."""
+    cfg = slm.DBConfig(n_peptides=3_000, seed=9, maxz=2, mods=[("M", 15.9949, 2), ("STY", 79.96633, 1)],
+                       max_mods_per_pep=2)
+    full = slm.SlmIndex(cfg)
+    eo = slm.SlmIndex(cfg, fragments=False)
+    assert full.has_fragments and not eo.has_fragments
+    assert len(full.buckets) == len(eo.buckets)
+    for a, b in zip(full.buckets, eo.buckets):
+        assert a.n_entries == b.n_entries and a.n_ions == b.n_ions
+        for f in ("mass", "seqid", "modnum", "sites", "seqs"):
+            assert np.array_equal(getattr(a, f), getattr(b, f)), f
+        assert b.bA.size == 0 and b.iA.size == 0
+    bp = eo.build_params()
+    assert (bp.maxz, bp.scale, bp.min_mass, bp.max_mass) == (2, 100, 500.0, 5000.0)
+    assert bp.n_mod_types == 2 and bp.modmass_all == 0
+    assert list(bp.mod_mass[:2]) == [int(float(np.float32(15.9949)) * 100.0), int(float(np.float32(79.96633)) * 100.0)]
