@@ -29,7 +29,7 @@ ORACLE = ROOT / "oracle"
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 OFFLOAD_ARCH = "gfx950"
 
-HOST_SOURCES = [CSRC / "host" / "slm_build.cpp"]
+HOST_SOURCES = [CSRC / "host" / "slm_build.cpp", CSRC / "host" / "ms2.cpp"]
 HIP_SOURCES = [
     CSRC / "hip" / "search.hip",
     CSRC / "hip" / "batched.hip",
@@ -37,6 +37,7 @@ HIP_SOURCES = [
     CSRC / "hip" / "mlp.hip",
     CSRC / "hip" / "mlp_capi.hip",
     CSRC / "hip" / "index_build.hip",
+    CSRC / "hip" / "preprocess.hip",
 ]
 HIP_HEADERS = sorted((CSRC / "hip").glob("*.h")) + sorted(INCLUDE.glob("*.h"))
 

@@ -140,6 +140,11 @@ class gx_db_params(C.Structure):
 GX_MAX_MOD_TYPES = 15
 
 
+class gx_pick_params(C.Structure):
+    _fields_ = [("base_int", C.c_int32), ("min_int", C.c_int32), ("qalen", C.c_int32)]
+
+
+
 class gx_index_build_params(C.Structure):
     _fields_ = [
         ("maxz", C.c_int32), ("scale", C.c_int32), ("min_mass", C.c_float), ("max_mass", C.c_float),
@@ -235,6 +240,13 @@ def host_lib() -> C.CDLL:
         lib.gx_spectra_generate.argtypes = [vp, C.POINTER(gx_spec_params), u64, C.POINTER(C.POINTER(gx_queries))]
         lib.gx_queries_destroy.argtypes = [C.POINTER(gx_queries)]
         lib.gx_queries_destroy.restype = None
+        lib.gx_spectra_generate_raw.argtypes = [vp, C.POINTER(gx_spec_params), u64,
+                                                C.POINTER(C.POINTER(gx_queries))]
+        lib.gx_ms2_read.argtypes = [C.c_char_p, C.c_int32, C.POINTER(C.POINTER(gx_queries))]
+        lib.gx_queries_pick.argtypes = [C.POINTER(gx_queries), C.POINTER(gx_pick_params),
+                                        C.POINTER(C.POINTER(gx_queries))]
+        lib.gx_pbin_write.argtypes = [C.c_char_p, C.POINTER(gx_queries)]
+        lib.gx_pbin_read.argtypes = [C.c_char_p, C.POINTER(C.POINTER(gx_queries))]
         _HOST = lib
     return _HOST
 
@@ -256,6 +268,10 @@ def hip_lib() -> C.CDLL:
         lib.gx_queries_upload.argtypes = [vp, C.POINTER(gx_query_batch), C.POINTER(vp)]
         lib.gx_queries_free.argtypes = [vp]
         lib.gx_queries_free.restype = None
+        lib.gx_queries_preprocess.argtypes = [vp, C.POINTER(gx_query_batch), C.POINTER(gx_pick_params),
+                                              C.POINTER(vp)]
+        lib.gx_queries_shape.argtypes = [vp, C.POINTER(C.c_uint32), C.POINTER(C.c_uint64), C.POINTER(C.c_float)]
+        lib.gx_queries_download.argtypes = [vp, vp, vp, vp]
         lib.gx_search_device.argtypes = [vp, vp, vp, C.POINTER(gx_search_params), C.POINTER(gx_search_stats)]
         lib.gx_results_download.argtypes = [vp, vp, vp, vp]
         lib.gx_search.argtypes = [vp, vp, C.POINTER(gx_query_batch), C.POINTER(gx_search_params), vp, vp,

@@ -62,6 +62,7 @@ struct gx_dqueries {
     uint64_t n_peaks = 0;
     uint32_t max_peaks = 0;
     bool has_dm = false;
+    float prep_ms = 0.0f;  // device time of gx_queries_preprocess (0 for uploaded batches)
     uint32_t *idx = nullptr;
     int32_t *moz = nullptr;
     int32_t *inten = nullptr;

@@ -16,6 +16,7 @@
 // reference uses an unstable std::sort (lbe_internal.cpp:189).
 #include "gicops_index.h"
 #include "gicops_amd.h"
+#include "pick.h"
 #include "slm_constants.h"
 
 #include <omp.h>
@@ -481,7 +482,7 @@ struct Peak {
 
 }  // namespace
 
-extern "C" int gx_spectra_generate(const gx_db *db, const gx_spec_params *p, uint64_t seed, gx_queries **out) {
+static int generate(const gx_db *db, const gx_spec_params *p, uint64_t seed, gx_queries **out, bool pick) {
     if (!db || !p || !out) return GX_ERR_INVLD_PTR;
     if (!db->finalized) return GX_ERR_NOT_READY;
     if (p->qalen < 1 || p->base_int < 1 || p->n_dm < 1 || p->n_dm > 4 || !(p->keep_frac >= 0 && p->keep_frac <= 1) ||
@@ -553,36 +554,17 @@ extern "C" int gx_spectra_generate(const gx_db *db, const gx_spec_params *p, uin
             }
             // raw integer peaks as readMS2 stores them (msquery.cpp:286-287):
             // m/z * scale and intensity (base peak 100) * 100, truncated
-            std::vector<Peak> raw;
-            raw.reserve(pk_mz.size());
+            std::vector<int32_t> rmz(pk_mz.size()), rin(pk_mz.size());
             for (size_t i = 0; i < pk_mz.size(); i++) {
-                Peak q;
-                q.mz = (int32_t)(pk_mz[i] * db->p.scale);
-                q.in = (int32_t)((pk_in[i] * (100.0 / imax)) * 100.0);
-                raw.push_back(q);
+                rmz[i] = (int32_t)(pk_mz[i] * db->p.scale);
+                rin[i] = (int32_t)((pk_in[i] * (100.0 / imax)) * 100.0);
             }
-            // pickpeaks (msquery.cpp:338-423): ascending intensity (ties by m/z),
-            // base peak -> base_int, the next QALEN-1 scaled by base_int/max and
-            // kept while >= min_int, stored in ascending-intensity order.
-            std::sort(raw.begin(), raw.end(),
-                      [](const Peak &x, const Peak &y) { return x.in != y.in ? x.in < y.in : x.mz < y.mz; });
             std::vector<Peak> &outp = specs[s];
             outp.clear();
-            const int sz = (int)raw.size();
-            if (sz > 0) {
-                const double factor = (double)p->base_int / raw[sz - 1].in;
-                raw[sz - 1].in = p->base_int;
-                int newlen = 1;
-                for (int j = sz - 2; j >= sz - p->qalen && j >= 0; j--) {
-                    raw[j].in = (int32_t)(raw[j].in * factor);
-                    if (raw[j].in >= p->min_int)
-                        newlen++;
-                    else
-                        break;
-                }
-                if (newlen >= p->qalen) newlen = p->qalen;
-                outp.assign(raw.end() - newlen, raw.end());
-            }
+            // pickpeaks (msquery.cpp:338-423), or the raw peaks as read
+            const int64_t sz = (int64_t)rmz.size();
+            const int64_t kept = pick ? pick_peaks(rin.data(), rmz.data(), sz, p->base_int, p->min_int, p->qalen) : sz;
+            for (int64_t i = sz - kept; i < sz; i++) outp.push_back(Peak{rmz[i], rin[i]});
             const float pm = b.mass[k];
             prec[s] = pm + kProton;
             chg[s] = p->charge_min + (int32_t)rng.below((uint32_t)(p->charge_max - p->charge_min + 1));
@@ -625,6 +607,14 @@ extern "C" int gx_spectra_generate(const gx_db *db, const gx_spec_params *p, uin
     q->idx[ns] = (uint32_t)off;
     *out = q;
     return GX_SUCCESS;
+}
+
+extern "C" int gx_spectra_generate(const gx_db *db, const gx_spec_params *p, uint64_t seed, gx_queries **out) {
+    return generate(db, p, seed, out, true);
+}
+
+extern "C" int gx_spectra_generate_raw(const gx_db *db, const gx_spec_params *p, uint64_t seed, gx_queries **out) {
+    return generate(db, p, seed, out, false);
 }
 
 extern "C" void gx_queries_destroy(gx_queries *q) {

@@ -16,7 +16,7 @@ from dataclasses import dataclass
 import numpy as np
 
 from . import _ffi as F
-from .slm import QueryBatch, SlmIndex
+from .slm import PickParams, QueryBatch, SlmIndex
 
 
 @dataclass
@@ -56,14 +56,37 @@ def stats_dict(st: F.gx_search_stats) -> dict:
 class DeviceQueries:
     """A query batch resident in HBM (dQueries, superstep3/kernel.cu:192-206)."""
 
-    def __init__(self, searcher: "Searcher", batch: QueryBatch, with_dm: bool = True):
+    def __init__(self, searcher: "Searcher", batch: QueryBatch, with_dm: bool = True, pick: PickParams | None = None):
+        """Uploads `batch` as it is, or, with `pick`, treats it as raw spectra
+        and picks their peaks on the GPU (gx_queries_preprocess)."""
         self._lib = searcher._lib
         self._s = searcher
         self.n_spectra = batch.n_spectra
         q = batch.as_ffi(with_dm)
         h = C.c_void_p()
-        F.check("gx_queries_upload", self._lib.gx_queries_upload(searcher._ctx, C.byref(q), C.byref(h)))
+        if pick is None:
+            F.check("gx_queries_upload", self._lib.gx_queries_upload(searcher._ctx, C.byref(q), C.byref(h)))
+        else:
+            pp = pick.to_ffi()
+            F.check("gx_queries_preprocess",
+                    self._lib.gx_queries_preprocess(searcher._ctx, C.byref(q), C.byref(pp), C.byref(h)))
         self._h = h
+
+    def shape(self) -> tuple[int, int, float]:
+        """(n_spectra, n_peaks, device ms of the GPU peak picking or 0)."""
+        ns, npk, ms = C.c_uint32(), C.c_uint64(), C.c_float()
+        F.check("gx_queries_shape", self._lib.gx_queries_shape(self._h, C.byref(ns), C.byref(npk), C.byref(ms)))
+        return ns.value, npk.value, ms.value
+
+    def peaks(self) -> tuple[np.ndarray, np.ndarray, np.ndarray]:
+        """(idx, moz, intensity) of the batch as the device holds it."""
+        ns, npk, _ = self.shape()
+        idx = np.zeros(ns + 1, np.uint32)
+        moz = np.zeros(npk, np.int32)
+        inten = np.zeros(npk, np.int32)
+        F.check("gx_queries_download", self._lib.gx_queries_download(
+            self._h, F.ptr(idx), F.ptr(moz) if npk else None, F.ptr(inten) if npk else None))
+        return idx, moz, inten
 
     def free(self) -> None:
         if getattr(self, "_h", None):
@@ -126,6 +149,10 @@ class Searcher:
 
     def upload(self, batch: QueryBatch, with_dm: bool = True) -> DeviceQueries:
         return DeviceQueries(self, batch, with_dm)
+
+    def preprocess(self, raw: QueryBatch, pick: PickParams | None = None, with_dm: bool = True) -> DeviceQueries:
+        """Pick the peaks of raw spectra on the GPU into a device batch."""
+        return DeviceQueries(self, raw, with_dm, pick or PickParams())
 
     def search_device(self, dq: DeviceQueries, params: SearchParams) -> dict:
         st = F.gx_search_stats()

@@ -229,6 +229,20 @@ class QueryBatch:
                           None if self.truth_bucket is None else self.truth_bucket[sel],
                           None if self.truth_psid is None else self.truth_psid[sel])
 
+    def as_gx_queries(self) -> F.gx_queries:
+        """A borrowed gx_queries view of this batch (valid while it lives)."""
+        q = F.gx_queries()
+        q.n_spectra, q.n_peaks = self.n_spectra, self.n_peaks
+
+        def pt(a, ct):
+            return C.cast(a.ctypes.data, C.POINTER(ct)) if a is not None and a.size else None
+
+        q.idx, q.moz, q.intensity = pt(self.idx, C.c_uint32), pt(self.moz, C.c_int32), pt(self.intensity, C.c_int32)
+        q.precurse, q.charges, q.rtimes = (pt(self.precurse, C.c_float), pt(self.charges, C.c_int32),
+                                           pt(self.rtimes, C.c_float))
+        q.dm = pt(self.dm, C.c_float)
+        return q
+
     def as_ffi(self, with_dm: bool = True) -> F.gx_query_batch:
         q = F.gx_query_batch()
         q.n_spectra = self.n_spectra
@@ -239,8 +253,40 @@ class QueryBatch:
         return q
 
 
-def generate_spectra(index: SlmIndex, cfg: SpecConfig) -> QueryBatch:
-    lib = F.host_lib()
+@dataclass
+class PickParams:
+    """Peak picking (gParams base_int, min_int, QALEN; argp.hpp:304-307)."""
+    base_int: int = 100_000
+    min_int: int = 1000
+    qalen: int = 100
+
+    def to_ffi(self) -> F.gx_pick_params:
+        p = F.gx_pick_params()
+        p.base_int, p.min_int, p.qalen = self.base_int, self.min_int, self.qalen
+        return p
+
+
+def _batch_from_gx(lib, out) -> QueryBatch:
+    """Copy a library-owned gx_queries into a QueryBatch and destroy it."""
+    try:
+        q = out.contents
+        n, npk = q.n_spectra, q.n_peaks
+
+        def cp(pp, k, dt):
+            if not pp:
+                return None
+            if k == 0:
+                return np.zeros(0, dtype=dt)
+            return np.ctypeslib.as_array(pp, shape=(k,)).astype(dt, copy=True)
+
+        return QueryBatch(cp(q.idx, n + 1, np.uint32), cp(q.moz, npk, np.int32), cp(q.intensity, npk, np.int32),
+                          cp(q.precurse, n, np.float32), cp(q.charges, n, np.int32), cp(q.rtimes, n, np.float32),
+                          cp(q.dm, n, np.float32), cp(q.truth_bucket, n, np.int32), cp(q.truth_psid, n, np.int32))
+    finally:
+        lib.gx_queries_destroy(out)
+
+
+def _spec_params(cfg: SpecConfig) -> F.gx_spec_params:
     p = F.gx_spec_params()
     p.n_spectra = cfg.n_spectra
     p.keep_frac, p.mz_sigma = cfg.keep_frac, cfg.mz_sigma
@@ -252,21 +298,52 @@ def generate_spectra(index: SlmIndex, cfg: SpecConfig) -> QueryBatch:
     p.n_dm = len(cfg.dm_choices)
     for i, d in enumerate(cfg.dm_choices):
         p.dm_choices[i] = d
+    return p
+
+
+def generate_raw_spectra(index: SlmIndex, cfg: SpecConfig) -> QueryBatch:
+    """The spectra of generate_spectra(index, cfg) before peak picking, as an
+    MS2 reader stores them (m/z*scale, intensity*100)."""
+    lib = F.host_lib()
+    p = _spec_params(cfg)
+    out = C.POINTER(F.gx_queries)()
+    F.check("gx_spectra_generate_raw", lib.gx_spectra_generate_raw(index._h, C.byref(p), cfg.seed, C.byref(out)))
+    return _batch_from_gx(lib, out)
+
+
+def read_ms2(path, scale: int = 100) -> QueryBatch:
+    """Raw spectra of an MS2 text file (gx_ms2_read)."""
+    lib = F.host_lib()
+    out = C.POINTER(F.gx_queries)()
+    F.check("gx_ms2_read", lib.gx_ms2_read(str(path).encode(), scale, C.byref(out)))
+    return _batch_from_gx(lib, out)
+
+
+def pick_peaks(batch: QueryBatch, pick: PickParams | None = None) -> QueryBatch:
+    """Peak picking on the CPU (gx_queries_pick)."""
+    lib = F.host_lib()
+    pp = (pick or PickParams()).to_ffi()
+    out = C.POINTER(F.gx_queries)()
+    F.check("gx_queries_pick", lib.gx_queries_pick(C.byref(batch.as_gx_queries()), C.byref(pp), C.byref(out)))
+    return _batch_from_gx(lib, out)
+
+
+def write_pbin(path, batch: QueryBatch) -> None:
+    lib = F.host_lib()
+    F.check("gx_pbin_write", lib.gx_pbin_write(str(path).encode(), C.byref(batch.as_gx_queries())))
+
+
+def read_pbin(path) -> QueryBatch:
+    lib = F.host_lib()
+    out = C.POINTER(F.gx_queries)()
+    F.check("gx_pbin_read", lib.gx_pbin_read(str(path).encode(), C.byref(out)))
+    return _batch_from_gx(lib, out)
+
+
+def generate_spectra(index: SlmIndex, cfg: SpecConfig) -> QueryBatch:
+    """Seeded synthetic spectra of index entries, peaks picked (gx_spectra_generate)."""
+    lib = F.host_lib()
+    p = _spec_params(cfg)
     out = C.POINTER(F.gx_queries)()
     F.check("gx_spectra_generate", lib.gx_spectra_generate(index._h, C.byref(p), cfg.seed, C.byref(out)))
-    try:
-        q = out.contents
-        n, npk = q.n_spectra, q.n_peaks
-
-        def cp(pp, k, dt):
-            if k == 0:
-                return np.zeros(0, dtype=dt)
-            return np.ctypeslib.as_array(pp, shape=(k,)).astype(dt, copy=True)
-
-        batch = QueryBatch(cp(q.idx, n + 1, np.uint32), cp(q.moz, npk, np.int32),
-                           cp(q.intensity, npk, np.int32), cp(q.precurse, n, np.float32),
-                           cp(q.charges, n, np.int32), cp(q.rtimes, n, np.float32), cp(q.dm, n, np.float32),
-                           cp(q.truth_bucket, n, np.int32), cp(q.truth_psid, n, np.int32))
-    finally:
-        lib.gx_queries_destroy(out)
-    return batch
+    return _batch_from_gx(lib, out)

@@ -206,6 +206,33 @@ int gx_index_download(gx_dindex *idx, int bucket, uint32_t *bA, uint32_t *iA);
 int gx_queries_upload(gx_ctx *ctx, const gx_query_batch *q, gx_dqueries **out);
 void gx_queries_free(gx_dqueries *q);
 
+/* Peak-picking parameters: gParams base_int (base_int x YAXISMULTIPLIER),
+ * min_int (base_int * cutoff + 0.5) (argp.hpp:304-307) and QALEN. */
+typedef struct gx_pick_params {
+    int32_t base_int; /* intensity of the most intense peak after normalisation (100000) */
+    int32_t min_int;  /* smallest kept normalised intensity (1000) */
+    int32_t qalen;    /* most intense peaks kept, 1..2048 (QALEN, 100) */
+} gx_pick_params;
+
+/* Preprocess a batch of raw spectra on the GPU into a device-resident query
+ * batch ready for gx_search_device, replacing s2::ArraySort +
+ * s2::computenewlens + s2::pickpeaks (superstep2/kernel.cu:100-422) with the
+ * CPU's MSQuery::pickpeaks semantics (msquery.cpp:338-423): per spectrum, the
+ * peaks sorted by intensity with the reference's key/value quicksort (so
+ * equal intensities end in its order, keyval.h:49-103), the most intense set
+ * to base_int, the next qalen-1 scaled by base_int/max and kept while
+ * >= min_int, stored in ascending-intensity order. `raw` holds integer m/z
+ * (m/z*scale) and intensity (intensity*100) as the MS2 reader stores them
+ * (msquery.cpp:286-287); its dm array, if any, is kept. */
+int gx_queries_preprocess(gx_ctx *ctx, const gx_query_batch *raw, const gx_pick_params *pp, gx_dqueries **out);
+
+/* Number of spectra and peaks of a device batch, the device time of its
+ * gx_queries_preprocess (sort, scan and compaction on the context stream,
+ * HIP events; 0 for an uploaded batch), and a copy of its CSR arrays to the
+ * host (idx: n_spectra+1, moz/intensity: n_peaks). Any pointer may be NULL. */
+int gx_queries_shape(const gx_dqueries *q, uint32_t *n_spectra, uint64_t *n_peaks, float *prep_ms);
+int gx_queries_download(const gx_dqueries *q, uint32_t *idx, int32_t *moz, int32_t *intensity);
+
 /* Search a device-resident batch and model its e-values; results stay on the
  * device until gx_results_download(). Replaces s3::search +
  * s4::processResults for one batch. */

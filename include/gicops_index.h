@@ -141,6 +141,35 @@ int gx_spectra_generate(const gx_db *db, const gx_spec_params *p, uint64_t seed,
                         gx_queries **out);
 void gx_queries_destroy(gx_queries *q);
 
+/* The same spectra before peak picking: every generated peak as the MS2
+ * reader would store it (m/z*scale, intensity*100, base peak 10000), in
+ * generation order. gx_queries_pick of this batch equals gx_spectra_generate's. */
+int gx_spectra_generate_raw(const gx_db *db, const gx_spec_params *p, uint64_t seed, gx_queries **out);
+
+/* ---------------------------------------------------------------------- */
+/* MS2 spectra, peak picking and the .pbin format                         */
+/* ---------------------------------------------------------------------- */
+
+/* Read an MS2 text file into raw spectra, as MSQuery::initialize parses it
+ * (msquery.cpp:170-330): "S" starts a spectrum, "Z <charge> <M+H>" sets
+ * charge (at least 1) and precursor, "I RTime <t>" the retention time (at
+ * least 0), H/D lines and empty lines are skipped, any other line is a peak
+ * "<m/z> <intensity>" stored as (int)(m/z*scale) and (int)(intensity*100).
+ * Data before the first S line is ignored; a spectrum without a Z line has
+ * charge 1 and precursor 0.01, without RTime retention time 0. dm is NULL. */
+int gx_ms2_read(const char *path, int32_t scale, gx_queries **out);
+
+/* Peak picking on the CPU: MSQuery::pickpeaks (msquery.cpp:338-423) applied
+ * to every spectrum of a raw batch (see gx_queries_preprocess for the rules). */
+int gx_queries_pick(const gx_queries *raw, const gx_pick_params *pp, gx_queries **out);
+
+/* The .pbin batch file of MSQuery::flushBinaryFile / readBINbatch
+ * (msquery.cpp:425-458, 791-832): per spectrum, native-endian f32 precursor,
+ * i32 charge, f32 retention time, i32 peak count n, then n i32 m/z and n i32
+ * intensities; no header. gx_pbin_read reads every spectrum of the file. */
+int gx_pbin_write(const char *path, const gx_queries *q);
+int gx_pbin_read(const char *path, gx_queries **out);
+
 #ifdef __cplusplus
 }
 #endif

@@ -431,3 +431,51 @@ int orc_search(const gx_bucket *buckets, int n_buckets, const gx_query_batch *q,
     }
     return status;
 }
+
+/* ---- peak picking (MSQuery::pickpeaks, msquery.cpp:338-423) ----------- */
+
+/* KeyVal_Parallel_Internal (keyval.h:49-103) without the OpenMP tasks: the
+ * recursion visits the same subranges with the same partition steps. */
+static void keyval_qsort(int32_t *key, int32_t *val, int64_t left, int64_t right) {
+    int64_t i = left, j = right;
+    const int32_t pivotkey = key[(left + right) / 2];
+    while (i <= j) {
+        while (key[i] < pivotkey) i++;
+        while (key[j] > pivotkey) j--;
+        if (i <= j) {
+            const int32_t tk = key[i], tv = val[i];
+            key[i] = key[j];
+            val[i] = val[j];
+            key[j] = tk;
+            val[j] = tv;
+            i++;
+            j--;
+        }
+    }
+    if (left < j) keyval_qsort(key, val, left, j);
+    if (i < right) keyval_qsort(key, val, i, right);
+}
+
+/* x86-64 cvttsd2si: truncation, INT32_MIN when NaN or out of range */
+static int32_t cvt_i32(double x) {
+    if (!(x > -2147483649.0 && x < 2147483648.0)) return INT32_MIN;
+    return (int32_t)x;
+}
+
+int64_t orc_pickpeaks(int32_t *intn, int32_t *mz, int64_t n, int32_t base_int, int32_t min_int, int32_t qalen) {
+    if (n <= 0) return 0;
+    keyval_qsort(intn, mz, 0, n - 1);
+    const double factor = (double)base_int / intn[n - 1];
+    intn[n - 1] = base_int;
+    int64_t newlen = 1;
+    for (int64_t j = n - 2; j >= n - qalen && j >= 0; j--) {
+        intn[j] = cvt_i32(intn[j] * factor);
+        if (intn[j] >= min_int)
+            newlen++;
+        else
+            break;
+    }
+    if (newlen >= qalen) newlen = qalen;
+    return newlen;
+}
+

@@ -16,7 +16,10 @@
  *   orc_tailfit                      textbook least squares on hand-built
  *                                    survival histograms, every branch
  *                                    (n = 2, 3, > 3, stt1 == end1, the
- *                                    replacement paths, the flat slice).
+ *                                    replacement paths, the flat slice);
+ *   orc_pickpeaks                    hand-traced partition orders of tied
+ *                                    intensities, numpy's sort for distinct
+ *                                    ones, the normalisation in closed form.
  */
 #ifndef GICOPS_ORACLE_H
 #define GICOPS_ORACLE_H
@@ -71,6 +74,13 @@ int orc_search(const gx_bucket *buckets, int n_buckets, const gx_query_batch *q,
 int orc_scorecard(const gx_bucket *b, const int32_t *moz, const int32_t *inten, uint32_t npk,
                   const gx_search_params *p, int32_t minlimit, int32_t maxlimit, uint16_t *bc, uint16_t *yc,
                   uint32_t *ibc, uint32_t *iyc);
+
+/* MSQuery::pickpeaks (msquery.cpp:338-423) on one raw spectrum, in place:
+ * KeyVal_Parallel's quicksort by intensity (keyval.h:49-103), the base peak
+ * set to base_int, the next qalen-1 scaled by base_int/max (int *= double,
+ * x86 conversion) while >= min_int. Returns the kept count; the kept peaks
+ * are the last ones of the arrays. */
+int64_t orc_pickpeaks(int32_t *intn, int32_t *mz, int64_t n, int32_t base_int, int32_t min_int, int32_t qalen);
 
 #ifdef __cplusplus
 }

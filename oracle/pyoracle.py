@@ -43,8 +43,34 @@ def lib() -> C.CDLL:
                                  C.POINTER(F.gx_search_params), vp, vp, vp, C.c_int]
         L.orc_scorecard.argtypes = [C.POINTER(F.gx_bucket), vp, vp, C.c_uint32, C.POINTER(F.gx_search_params),
                                     C.c_int32, C.c_int32, vp, vp, vp, vp]
+        L.orc_pickpeaks.argtypes = [vp, vp, C.c_int64, C.c_int32, C.c_int32, C.c_int32]
+        L.orc_pickpeaks.restype = C.c_int64
         _LIB = L
     return _LIB
+
+
+def pickpeaks_one(intensity, mz, base_int: int = 100_000, min_int: int = 1000, qalen: int = 100):
+    """One raw spectrum -> (sorted intensities, sorted m/z, kept count); the
+    kept peaks are the last `kept` entries."""
+    intn = np.array(intensity, dtype=np.int32)
+    m = np.array(mz, dtype=np.int32)
+    k = lib().orc_pickpeaks(intn.ctypes.data if intn.size else None, m.ctypes.data if m.size else None,
+                            len(intn), base_int, min_int, qalen)
+    return intn, m, int(k)
+
+
+def pickpeaks(batch, base_int: int = 100_000, min_int: int = 1000, qalen: int = 100):
+    """Peak picking of every spectrum of a raw batch -> (idx, moz, intensity)."""
+    idx = [0]
+    moz, inten = [], []
+    for s in range(batch.n_spectra):
+        a, b = int(batch.idx[s]), int(batch.idx[s + 1])
+        i, m, k = pickpeaks_one(batch.intensity[a:b], batch.moz[a:b], base_int, min_int, qalen)
+        inten.append(i[len(i) - k:])
+        moz.append(m[len(m) - k:])
+        idx.append(idx[-1] + k)
+    cat = (lambda xs: np.concatenate(xs).astype(np.int32) if xs else np.zeros(0, np.int32))
+    return np.array(idx, dtype=np.uint32), cat(moz), cat(inten)
 
 
 def lower_bound(a: np.ndarray, v: int) -> int:

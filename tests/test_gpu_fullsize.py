@@ -107,3 +107,25 @@ def test_full_size_device_build(big_index):
         assert np.array_equal(bA, b.bA), f"bucket {i}: bA differs"
         assert np.array_equal(iA, b.iA), f"bucket {i}: iA differs"
     gs.close()
+
+
+@pytest.mark.parametrize("workload", ["open2m", "uneven"])
+def test_full_size_preprocess(big_index, big_searcher, workload):
+    """Raw spectra of the full-size workloads picked on the GPU equal the host
+    picking of the same raw spectra (and the generator's picked batch)."""
+    sc = slm.SPEC_PRESETS[workload]
+    say(f"case: full-size {workload} peak picking: {sc.n_spectra} raw spectra")
+    raw = slm.generate_raw_spectra(big_index, sc)
+    pick = slm.PickParams(sc.base_int, sc.min_int, sc.qalen)
+    t0 = time.perf_counter()
+    host = slm.pick_peaks(raw, pick)
+    t_host = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    dq = big_searcher.preprocess(raw, pick)
+    t_call = time.perf_counter() - t0
+    ns, npk, dev_ms = dq.shape()
+    say(f"  {raw.n_peaks} raw peaks -> {npk} kept; GPU {dev_ms:.2f} ms on the device "
+        f"({t_call * 1e3:.0f} ms with the upload), host (all cores) {t_host * 1e3:.0f} ms")
+    idx, moz, inten = dq.peaks()
+    assert np.array_equal(idx, host.idx) and np.array_equal(moz, host.moz) and np.array_equal(inten, host.intensity)
+    dq.free()
