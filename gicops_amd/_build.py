@@ -38,6 +38,7 @@ HIP_SOURCES = [
     CSRC / "hip" / "mlp_capi.hip",
     CSRC / "hip" / "index_build.hip",
     CSRC / "hip" / "preprocess.hip",
+    CSRC / "hip" / "partials.hip",
 ]
 HIP_HEADERS = sorted((CSRC / "hip").glob("*.h")) + sorted(INCLUDE.glob("*.h"))
 

@@ -152,6 +152,15 @@ class gx_index_build_params(C.Structure):
     ]
 
 
+class gx_partial(C.Structure):
+    _fields_ = [("key", C.c_uint64), ("eq", C.c_uint32), ("cpsms", C.c_uint32), ("sharedions", C.c_uint16),
+                ("totalions", C.c_uint16), ("reserved", C.c_uint32)]
+
+
+PARTIAL_DTYPE = np.dtype([("key", np.uint64), ("eq", np.uint32), ("cpsms", np.uint32), ("sharedions", np.uint16),
+                          ("totalions", np.uint16), ("reserved", np.uint32)])
+
+
 class gx_spec_params(C.Structure):
     _fields_ = [
         ("n_spectra", C.c_uint32), ("keep_frac", C.c_float), ("mz_sigma", C.c_float),
@@ -272,6 +281,9 @@ def hip_lib() -> C.CDLL:
                                               C.POINTER(vp)]
         lib.gx_queries_shape.argtypes = [vp, C.POINTER(C.c_uint32), C.POINTER(C.c_uint64), C.POINTER(C.c_float)]
         lib.gx_queries_download.argtypes = [vp, vp, vp, vp]
+        lib.gx_partials_export.argtypes = [vp, vp, vp, vp, vp, vp, C.c_int]
+        lib.gx_partials_merge.argtypes = [vp, C.c_int, C.POINTER(vp), C.POINTER(vp), C.c_uint32,
+                                          C.POINTER(gx_search_params), C.c_int, vp, vp, C.POINTER(C.c_float)]
         lib.gx_search_device.argtypes = [vp, vp, vp, C.POINTER(gx_search_params), C.POINTER(gx_search_stats)]
         lib.gx_results_download.argtypes = [vp, vp, vp, vp]
         lib.gx_search.argtypes = [vp, vp, C.POINTER(gx_query_batch), C.POINTER(gx_search_params), vp, vp,

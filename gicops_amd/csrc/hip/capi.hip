@@ -404,6 +404,8 @@ int search_plain(gx_ctx *c, const gx_dindex *idx, gx_dqueries *q, const gx_searc
     GX_HIP(hipEventRecord(c->ev1, c->stream));
     GX_HIP(hipStreamSynchronize(c->stream));
     q->searched = true;
+    q->batched_state = false;
+    q->searched_index = idx;
     if (stats) {
         unsigned long long h[ST_N];
         read_stats(c, h);
@@ -610,6 +612,9 @@ int search_batched(gx_ctx *c, const gx_dindex *idx, gx_dqueries *q, const gx_sea
         GX_HIP(hipGraphLaunch(q->graph_exec, sm));
         GX_HIP(hipEventRecord(c->ev3, sm));
         GX_HIP(hipStreamSynchronize(sm));
+        q->searched = true;
+        q->batched_state = true;
+        q->searched_index = idx;
         if (stats) {
             unsigned long long h[ST_N];
             read_stats(c, h);
@@ -716,6 +721,8 @@ int search_batched(gx_ctx *c, const gx_dindex *idx, gx_dqueries *q, const gx_sea
     GX_HIP(hipEventRecord(c->ev3, sm));
     GX_HIP(hipStreamSynchronize(sm));
     q->searched = true;
+    q->batched_state = true;
+    q->searched_index = idx;
     if (stats) {
         unsigned long long h[ST_N];
         read_stats(c, h);

@@ -71,6 +71,8 @@ struct gx_dqueries {
     gx_psm *psm = nullptr;
     uint32_t *hist = nullptr;  // per-spectrum survival histograms (batched: always; plain: on request)
     bool searched = false;
+    bool batched_state = false;               // the last search was batched: best_key/eq/shp, hist, cpsms hold it
+    const gx_dindex *searched_index = nullptr;  // the index of the last search
     // batched engine workspace, sized on first use
     int nb = 0;
     int32_t *lq = nullptr, *li = nullptr;

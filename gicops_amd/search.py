@@ -224,3 +224,113 @@ class Searcher:
             self.close()
         except Exception:
             pass
+
+
+def shard_bounds(n_entries: int, n_shards: int) -> list[tuple[int, int]]:
+    """Contiguous psid ranges [a, e) of one bucket's mass-sorted entries, one per shard."""
+    return [(n_entries * k // n_shards, n_entries * (k + 1) // n_shards) for k in range(n_shards)]
+
+
+class ShardedSearcher:
+    """The index-sharded search on one GPU: every bucket's mass-sorted entries
+    split into `n_shards` contiguous psid ranges, each shard's fragment-ion
+    index built on the GPU (gx_index_build), every batch searched against each
+    shard in turn, and the shards' partial results (gx_partials_export)
+    merged into PSMs with the tail fit on the summed histograms
+    (gx_partials_merge). The merge of contiguous ranges equals the unsharded
+    search exactly. This is what an index-sharded multi-GPU search does per
+    rank (the reference's MPI model, lbe_internal.cpp:75-103 and
+    dslim_score.cpp:199-387), with the exchange of partials through host
+    memory."""
+
+    def __init__(self, index: SlmIndex, n_shards: int, device: int = 0):
+        if n_shards < 1:
+            raise ValueError("n_shards >= 1")
+        self._lib = F.hip_lib()
+        ctx = C.c_void_p()
+        F.check("gx_ctx_create", self._lib.gx_ctx_create(device, C.byref(ctx)))
+        self._ctx = ctx
+        self.index = index
+        self.n_shards = n_shards
+        self._children: list = []
+        self._didx: list = []
+        self.offsets: list[np.ndarray] = []
+        bp = index.build_params()
+        nb = len(index.buckets)
+        for k in range(n_shards):
+            views = (F.gx_bucket * nb)()
+            off = np.zeros(nb, np.uint32)
+            for i, b in enumerate(index.buckets):
+                a, e = shard_bounds(b.n_entries, n_shards)[k]
+                v = views[i]
+                v.peplen, v.n_peps, v.speclen, v.nbins = b.peplen, b.n_peps, b.speclen, b.nbins
+                v.n_entries = e - a
+                v.n_ions = (e - a) * b.speclen
+                v.seqs = b.raw.seqs
+                if e > a:
+                    v.mass = b.mass[a:].ctypes.data
+                    v.seqid = b.seqid[a:].ctypes.data
+                    v.modnum = b.modnum[a:].ctypes.data
+                    v.sites = b.sites[a:].ctypes.data
+                off[i] = a
+            d = C.c_void_p()
+            st = self._lib.gx_index_build(ctx, views, nb, C.byref(bp), C.byref(d))
+            if st:
+                self.close()
+                F.check("gx_index_build", st)
+            self._didx.append(d)
+            self.offsets.append(off)
+
+    def upload(self, batch: QueryBatch, with_dm: bool = True) -> DeviceQueries:
+        return DeviceQueries(self, batch, with_dm)
+
+    def search(self, batch: QueryBatch, params: SearchParams, want_hist: bool = False):
+        """Search every shard, exchange the partials through host memory and
+        merge -> (psms, merged hist | None, stats with per-shard and merge times)."""
+        if params.engine != F.GX_ENGINE_BATCHED:
+            raise ValueError("the sharded search exports the batched engine's partials")
+        dq = self.upload(batch)
+        n = batch.n_spectra
+        recs = [np.zeros(n, F.PARTIAL_DTYPE) for _ in range(self.n_shards)]
+        hists = [np.zeros((n, F.GX_HIST_BINS), np.uint32) for _ in range(self.n_shards)]
+        p = params.to_ffi()
+        shard_ms, ions = [], 0
+        for k, d in enumerate(self._didx):
+            st = F.gx_search_stats()
+            F.check("gx_search_device", self._lib.gx_search_device(self._ctx, d, dq._h, C.byref(p), C.byref(st)))
+            shard_ms.append(st.total_ms)
+            ions += st.ions
+            F.check("gx_partials_export", self._lib.gx_partials_export(
+                self._ctx, d, dq._h, F.ptr(self.offsets[k]), F.ptr(recs[k]), F.ptr(hists[k]), 0))
+        dq.free()
+        psm, hist, ms = self.merge(recs, hists, params, want_hist)
+        return psm, hist, {"shard_ms": shard_ms, "merge_ms": ms, "ions": ions}
+
+    def merge(self, recs, hists, params: SearchParams, want_hist: bool = False):
+        """gx_partials_merge over host partials -> (psms, hist | None, device ms)."""
+        n = len(recs[0])
+        k = len(recs)
+        psm = np.zeros(n, dtype=F.PSM_DTYPE)
+        hist = np.zeros((n, F.GX_HIST_BINS), np.uint32) if want_hist else None
+        rp = (C.c_void_p * k)(*[r.ctypes.data for r in recs])
+        hp = (C.c_void_p * k)(*[h.ctypes.data for h in hists])
+        ms = C.c_float()
+        p = params.to_ffi()
+        F.check("gx_partials_merge", self._lib.gx_partials_merge(self._ctx, k, rp, hp, n, C.byref(p), 0, F.ptr(psm),
+                                                                 F.ptr(hist), C.byref(ms)))
+        return psm, hist, ms.value
+
+    def close(self) -> None:
+        for d in self._didx:
+            self._lib.gx_index_free(d)
+        self._didx = []
+        if getattr(self, "_ctx", None):
+            self._lib.gx_ctx_destroy(self._ctx)
+            self._ctx = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+

@@ -269,6 +269,48 @@ typedef struct gx_tune_result {
 int gx_autotune(gx_ctx *ctx, const gx_dindex *idx, gx_dqueries *sample, const gx_search_params *p,
                 gx_tune_result *out);
 
+/* ---- index-sharded search: partial results and their merge ---------- */
+/* One index shard's result for one spectrum, the record an index-sharded
+ * search exchanges before the e-value. It replaces the reference's partRes
+ * + u16 histogram buffer (slm_dsts.h:856-934, 1005-1040; filled by
+ * dslim_query.cpp:973-1000 and expeRT::StoreIResults, expeRT.cpp:714-806),
+ * which is lossy above 65535 counts and limited to 128 histogram bins; the
+ * whole 802-bin u32 histogram of the spectrum travels beside it. */
+typedef struct gx_partial {
+    uint64_t key;        /* top hit: f32 hyperscore bits << 32 | ~(bucket << 26 | psid); 0 = none.
+                            A larger key is the better hit (ties: smaller bucket, then psid) */
+    uint32_t eq;         /* entries of this shard at the top hyperscore */
+    uint32_t cpsms;      /* this shard's candidates */
+    uint16_t sharedions; /* bc+yc of this shard's top hit */
+    uint16_t totalions;  /* speclen of its bucket */
+    uint32_t reserved;
+} gx_partial;
+
+/* Export the partial results of the last (batched-engine) search of `q` on
+ * index shard `idx`: one gx_partial per spectrum into `rec` and the
+ * n_spectra x GX_HIST_BINS survival histograms into `hist` (either may be
+ * NULL). psid_offset[bucket] (NULL = 0) is added to the shard's psids, so
+ * that a shard holding entries [off, off+n) of a bucket's mass-sorted list
+ * reports global psids. on_device: the destinations are device memory of
+ * the context's GPU (e.g. a collective's send buffer), else host memory.
+ * No reference counterpart for the export itself; the reference writes its
+ * partials to files (dslim_query.cpp:1005-1012, dslim_comm.cpp). */
+int gx_partials_export(gx_ctx *ctx, const gx_dindex *idx, const gx_dqueries *q, const uint32_t *psid_offset,
+                       gx_partial *rec, uint32_t *hist, int on_device);
+
+/* Merge n_parts shards' partials of the same n_spectra spectra into PSMs,
+ * replacing DSLIM_Score::CombineResults (dslim_score.cpp:199-387): the
+ * histograms and candidate counts add, the top hit is the largest key (the
+ * unsharded tie rule, so the merge of contiguous psid ranges equals the
+ * unsharded search exactly; the reference prefers the first node), and the
+ * tail fit and e-value run on the merged histogram. rec[k]/hist[k] are device
+ * memory (on_device = 1) or host memory. Writes psms (host) and, if not
+ * NULL, the merged histograms (host); device_ms (may be NULL) receives the
+ * merge kernel's time. */
+int gx_partials_merge(gx_ctx *ctx, int n_parts, const gx_partial *const *rec, const uint32_t *const *hist,
+                      uint32_t n_spectra, const gx_search_params *p, int on_device, gx_psm *psms,
+                      uint32_t *merged_hist, float *device_ms);
+
 /* Text of the last failure on this thread: for GX_ERR_DEVICE the call, its
  * source line and the HIP error string (the reference prints the CUDA error
  * and exits, cuda/driver.hpp:41-48); for a rejected search argument, which

@@ -19,7 +19,7 @@ import pytest
 
 from gicops_amd import _ffi as F
 from gicops_amd import slm
-from gicops_amd.search import Searcher, SearchParams
+from gicops_amd.search import Searcher, SearchParams, ShardedSearcher
 from oracle import pyoracle as O
 
 from test_gpu_parity import compare  # noqa: E402
@@ -129,3 +129,19 @@ def test_full_size_preprocess(big_index, big_searcher, workload):
     idx, moz, inten = dq.peaks()
     assert np.array_equal(idx, host.idx) and np.array_equal(moz, host.moz) and np.array_equal(inten, host.intensity)
     dq.free()
+
+
+def test_full_size_sharded(big_index, big_searcher):
+    """The open workload over the 2M-peptide index in 4 index shards, merged:
+    identical to the unsharded search."""
+    say("case: full-size open2m, 4 index shards merged")
+    batch = slm.generate_spectra(big_index, slm.SPEC_PRESETS["open2m"])
+    p = SearchParams(dM=500.0)
+    want, want_h, st = big_searcher.search(batch, p, want_hist=True)
+    sh = ShardedSearcher(big_index, 4)
+    got, got_h, sst = sh.search(batch, p, want_hist=True)
+    sh.close()
+    say(f"  unsharded {st['total_ms']:.1f} ms; shards " + ", ".join(f"{x:.1f}" for x in sst["shard_ms"]) +
+        f" ms; merge {sst['merge_ms']:.2f} ms")
+    assert sst["ions"] == st["ions"]
+    assert got.tobytes() == want.tobytes() and np.array_equal(got_h, want_h)
