@@ -55,6 +55,7 @@ struct ModTable {
     uint32_t mod_mass[kMaxModTypes] = {};  // (uint)(mass*scale)
     int per_pep[kMaxModTypes] = {};
     int type_of[26];                       // -1 when the residue carries no variable mod
+    int nterm_type = -1;                   // type of the peptide N-terminus ("^"), -1 if none
 };
 
 // UTILS_CalculateModMass, utils.cpp:306-331. With all_mods == false only the
@@ -183,6 +184,16 @@ void mod_list(const char *seq, int len, const ModTable &mt, int conditions[], in
                  modnum + (1u << (4 * mods_seen)) * (uint32_t)(1 + t), true, out, seqid, scale, all_mods);
         conditions[t] += 1;
     }
+    // N-terminal modification (an extension: the reference grammar has none):
+    // a site-0 modification of any residue, tried after the residue's own one;
+    // one modification per site, so it excludes the residue's at position 0
+    const int u = mt.nterm_type;
+    if (letter == 0 && u != -1 && conditions[u] > 0 && mods_seen < kMaxModsPerPep) {
+        conditions[u] -= 1;
+        mod_list(seq, len, mt, conditions, total - 1, letter + 1, mods_seen + 1, sites | 1ull,
+                 modnum + (1u << (4 * mods_seen)) * (uint32_t)(1 + u), true, out, seqid, scale, all_mods);
+        conditions[u] += 1;
+    }
     mod_list(seq, len, mt, conditions, total, letter + 1, mods_seen, sites, modnum, false, out, seqid, scale,
              all_mods);
 }
@@ -239,6 +250,14 @@ extern "C" int gx_db_create(const gx_db_params *p, gx_db **out) {
     for (int t = 0; t < p->n_mod_types; t++) {
         db->mt.mod_mass[t] = (uint32_t)(p->mods[t].mass * (double)p->scale);
         db->mt.per_pep[t] = p->mods[t].max_per_pep;
+        if (std::strncmp(p->mods[t].residues, "^", 8) == 0) {  // the peptide N-terminus
+            if (db->mt.nterm_type != -1) {
+                delete db;
+                return GX_ERR_INVLD_MOD;
+            }
+            db->mt.nterm_type = t;
+            continue;
+        }
         for (const char *r = p->mods[t].residues; *r && r < p->mods[t].residues + 8; r++) {
             if (!is_residue(*r)) {
                 delete db;
@@ -390,13 +409,30 @@ static int finalize(gx_db *db, bool fragments) {
     const int saved = omp_get_max_threads();
     omp_set_num_threads(nthr);
     int st = GX_SUCCESS;
-    try {
-        for (auto &b : db->buckets) {
-            st = build_bucket(db, b, fragments);
-            if (st != GX_SUCCESS) break;
+    if (fragments) {  // buckets one after another, each fragment pass parallel
+        try {
+            for (auto &b : db->buckets) {
+                st = build_bucket(db, b, fragments);
+                if (st != GX_SUCCESS) break;
+            }
+        } catch (const std::bad_alloc &) {
+            st = GX_ERR_BAD_MEM_ALLOC;
         }
-    } catch (const std::bad_alloc &) {
-        st = GX_ERR_BAD_MEM_ALLOC;
+    } else {  // entries only: the buckets in parallel
+        const int nb = (int)db->buckets.size();
+#pragma omp parallel for schedule(dynamic, 1)
+        for (int i = 0; i < nb; i++) {
+            int r;
+            try {
+                r = build_bucket(db, db->buckets[i], false);
+            } catch (const std::bad_alloc &) {
+                r = GX_ERR_BAD_MEM_ALLOC;
+            }
+            if (r != GX_SUCCESS) {
+#pragma omp critical
+                st = r;
+            }
+        }
     }
     omp_set_num_threads(saved);
     if (st == GX_SUCCESS) {

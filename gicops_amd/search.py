@@ -147,6 +147,11 @@ class Searcher:
                                             F.ptr(iA) if iA.size else None))
         return bA, iA
 
+    def host_index(self) -> "HostIndex":
+        """A host copy of the device index: the host index's entries with the
+        device's bA/iA downloaded (for CPU-side checks of a GPU-built index)."""
+        return HostIndex(self)
+
     def upload(self, batch: QueryBatch, with_dm: bool = True) -> DeviceQueries:
         return DeviceQueries(self, batch, with_dm)
 
@@ -224,6 +229,26 @@ class Searcher:
             self.close()
         except Exception:
             pass
+
+
+class HostIndex:
+    """Buckets with the host index's entry arrays and the device index's
+    bA/iA copied to the host: `raw_array` (gx_bucket[]) and `buckets` have
+    the shape SlmIndex has."""
+
+    def __init__(self, searcher: "Searcher"):
+        index = searcher.index
+        self._index = index
+        self.arrays = []
+        self.buckets = index.buckets
+        self.raw_array = (F.gx_bucket * len(index.buckets))()
+        for i, b in enumerate(index.buckets):
+            bA, iA = searcher.index_arrays(i)
+            self.arrays.append((bA, iA))
+            r = self.raw_array[i]
+            C.memmove(C.byref(r), C.byref(b.raw), C.sizeof(F.gx_bucket))
+            r.bA = bA.ctypes.data if bA.size else None
+            r.iA = iA.ctypes.data if iA.size else None
 
 
 def shard_bounds(n_entries: int, n_shards: int) -> list[tuple[int, int]]:

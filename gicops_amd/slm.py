@@ -10,6 +10,9 @@ Workload presets (`DB_PRESETS`, `SPEC_PRESETS`) name the synthetic workloads:
   open2m  - 2M peptides + Met oxidation (<=2), 100k spectra, dM +-500 Da
   uneven  - the open2m index, 200k spectra with power-law peak counts over
             20-2000 and dM drawn from {10, 100, 500} per spectrum
+  c4      - 50M peptides + Met oxidation (<=2) + N-terminal acetylation,
+            <=2 modifications per peptide (~244M entries, ~12.3G ions; the
+            fragment arrays are built on the GPU), 1M spectra of the uneven mix
 """
 from __future__ import annotations
 
@@ -64,6 +67,9 @@ DB_PRESETS = {
     "tiny": DBConfig(n_peptides=2_000, seed=7),
     "small": DBConfig(n_peptides=50_000, seed=0),
     "open2m": DBConfig(n_peptides=2_000_000, seed=1, mods=[("M", 15.9949, 2)], max_mods_per_pep=2),
+    # 50M peptides, Met oxidation (<= 2) and N-terminal acetylation (an extension), <= 2 mods per peptide:
+    # ~244M entries, ~12.3G ions; build it with fragments=False and let the GPU build the fragments
+    "c4": DBConfig(n_peptides=50_000_000, seed=4, mods=[("M", 15.9949, 2), ("^", 42.0106, 1)], max_mods_per_pep=2),
 }
 
 SPEC_PRESETS = {
@@ -74,6 +80,9 @@ SPEC_PRESETS = {
     "uneven": SpecConfig(n_spectra=200_000, seed=3, peak_mode=F.GX_PEAKS_TOTAL_POWER, peaks_min=20,
                          peaks_max=2000, power_alpha=2.0, charge_min=2, charge_max=3, qalen=2048,
                          dm_choices=(10.0, 100.0, 500.0)),
+    "c4": SpecConfig(n_spectra=1_000_000, seed=5, peak_mode=F.GX_PEAKS_TOTAL_POWER, peaks_min=20,
+                     peaks_max=2000, power_alpha=2.0, charge_min=2, charge_max=3, qalen=2048,
+                     dm_choices=(10.0, 100.0, 500.0)),
 }
 
 

@@ -35,7 +35,10 @@ extern "C" {
  * mods.cpp:63-95): residues it applies to, mass in Da (stored as
  * (uint)(mass*scale)), and at most `max_per_pep` sites of this type. */
 typedef struct gx_mod_spec {
-    char residues[8];     /* NUL-terminated residue letters, e.g. "M" */
+    char residues[8];     /* NUL-terminated residue letters, e.g. "M"; "^" alone is the
+                             peptide N-terminus (an extension, e.g. acetylation +42.0106):
+                             a site-0 modification of any residue, tried after that
+                             residue's own modification and exclusive with it */
     float mass;           /* Da, e.g. 15.9949 */
     int32_t max_per_pep;  /* per-type limit (aa_per_peptide) */
 } gx_mod_spec;

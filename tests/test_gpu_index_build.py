@@ -5,7 +5,7 @@ Bar: bA and iA of every bucket bit-identical to the CPU build (which the
 oracle tests pin by closed forms and brute force), and searches over the
 device-built index byte-identical to searches over the uploaded one.
 Covered: unmodified peptides; variable modifications of several types with
-maxz 3 and every modification's mass in the precursor; a narrow precursor
+maxz 3 and every modification's mass in the precursor; N-terminal acetylation; a narrow precursor
 range that drops entries (zeroed fragments in bin 0); an entries-only host
 index (no host fragment arrays at all); bad build parameters.
 """
@@ -26,7 +26,9 @@ MODS3 = slm.DBConfig(n_peptides=30_000, seed=5, maxz=3,
                      max_mods_per_pep=3, modmass_all=True)
 NARROW = slm.DBConfig(n_peptides=20_000, seed=11, maxz=2, min_mass=900.0, max_mass=2500.0,
                       mods=[("M", 15.9949, 1)], max_mods_per_pep=1)
-CASES = {"small": slm.DB_PRESETS["small"], "mods3_z3": MODS3, "narrow_mass": NARROW}
+NTERM = slm.DBConfig(n_peptides=20_000, seed=17, mods=[("M", 15.9949, 2), ("^", 42.0106, 1)],
+                     max_mods_per_pep=3)
+CASES = {"small": slm.DB_PRESETS["small"], "mods3_z3": MODS3, "narrow_mass": NARROW, "nterm_acetyl": NTERM}
 
 
 def say(msg):

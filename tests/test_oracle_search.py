@@ -181,3 +181,33 @@ def test_entries_only_index_and_build_params():
     assert (bp.maxz, bp.scale, bp.min_mass, bp.max_mass) == (2, 100, 500.0, 5000.0)
     assert bp.n_mod_types == 2 and bp.modmass_all == 0
     assert list(bp.mod_mass[:2]) == [int(float(np.float32(15.9949)) * 100.0), int(float(np.float32(79.96633)) * 100.0)]
+
+
+def test_nterm_modification_variants_and_fragments():
+    """The "^" (N-terminus) modification: a site-0 modification of any residue,
+    exclusive with the residue's own; it moves every b-ion and no y-ion."""
+    mods = [("M", 15.9949, 2), ("^", 42.0106, 1)]
+    idx = slm.SlmIndex(slm.DBConfig(min_len=8, max_len=8, mods=mods, max_mods_per_pep=3),
+                       peptides=["AMAMAMAK", "MAAAAAAK"])
+    b = idx.buckets[0]
+    seqs = [b.sequence(k) for k in range(b.n_entries)]
+    # AMAMAMAK: 7 oxidation subsets (<= 2 of 3 sites) x acetyl yes/no; MAAAAAAK: none, ox0, ac0
+    assert seqs.count("AMAMAMAK") == 14 and seqs.count("MAAAAAAK") == 3
+    a_sites = sorted(int(s) for s, q in zip(b.sites, seqs) if q == "AMAMAMAK")
+    ox = [0, 2, 8, 32, 2 | 8, 2 | 32, 8 | 32]
+    assert a_sites == sorted(ox + [x | 1 for x in ox])
+    m_sites = sorted(int(s) for s, q in zip(b.sites, seqs) if q == "MAAAAAAK")
+    assert m_sites == [0, 1, 1]
+    m_mod = sorted(int(n) for s, n, q in zip(b.sites, b.modnum, seqs) if q == "MAAAAAAK" and s)
+    assert m_mod == [1, 2]  # type 0 (oxidation) and type 1 (acetyl) at site 0
+    inv = invert(b)
+    L, n = 8, 7
+    base = next(k for k in range(b.n_entries) if seqs[k] == "AMAMAMAK" and b.sites[k] == 0)
+    acet = next(k for k in range(b.n_entries) if seqs[k] == "AMAMAMAK" and b.sites[k] == 1)
+    f0 = inv[base * b.speclen:(base + 1) * b.speclen].astype(np.int64)
+    f1 = inv[acet * b.speclen:(acet + 1) * b.speclen].astype(np.int64)
+    assert np.all(np.isin(f1[:n] - f0[:n], [4201, 4202]))  # b-ions + 42.0106 Da
+    assert np.array_equal(f1[n:], f0[n:])                  # y-ions unchanged
+    # precursor: the first modification's mass only (reference quirk) -> + 42.0106
+    assert abs(float(b.mass[acet]) - float(b.mass[base]) - 42.0106) < 2e-3
+
