@@ -74,7 +74,12 @@ struct ItemArgs {
     const uint64_t *pos_off;
     const uint32_t *pos;
     uint32_t chunk_s0, chunk_s1;  // spectra of the chunk
-    const uint32_t *order;   // the chunk's spectra in the order they are handed out
+    // the chunk's work units in the order they are handed out: a spectrum, or
+    // a part of a heavy spectrum (a bucket range) whose results go to a slot
+    // of the part scratch and are folded afterwards
+    const uint32_t *unit_s;  // spectrum of each unit
+    const uint32_t *unit_br; // b0 | b1 << 8 | slot << 16; slot 0xffff = the whole spectrum
+    uint32_t unit0, n_units; // this chunk's units
     uint32_t *work_counter;  // this chunk's counter
     uint32_t *cand_scratch;  // per block overflow candidates
     int tile_entries_max;
@@ -88,8 +93,23 @@ struct ItemArgs {
     unsigned long long *best_key;  // top hit: h bits << 32 | ~(b << 26 | psid)
     uint32_t *best_eq;       // candidates with the top hyperscore
     uint32_t *best_shp;      // shared ions of the top hit
+    // results of the parts of heavy spectra, by slot
+    unsigned long long *part_key;
+    uint32_t *part_eq, *part_shp, *part_cps, *part_hist;
     unsigned long long *stats;
 };
+
+// folding of the parts of the chunk's heavy spectra into their results
+struct FoldArgs {
+    const uint32_t *f_s, *f_slot0, *f_n;  // spectrum, first slot and number of parts
+    uint32_t n_folds;
+    const unsigned long long *part_key;
+    const uint32_t *part_eq, *part_shp, *part_cps, *part_hist;
+    uint32_t *hist, *cpsms, *best_eq, *best_shp;
+    unsigned long long *best_key;
+};
+
+constexpr uint32_t kWholeUnit = 0xffffu;  // unit slot of a spectrum searched whole
 
 struct EvalArgs {
     const DevBucket *buckets;
@@ -113,5 +133,6 @@ hipError_t launch_scan(const PlanArgs &a, hipStream_t s);
 hipError_t launch_plan_slices(const PlanArgs &a, hipStream_t s);
 hipError_t launch_search_spectra(int config, const ItemArgs &a, int n_blocks, hipStream_t s);
 hipError_t launch_evalue(const EvalArgs &a, hipStream_t s);
+hipError_t launch_fold(const FoldArgs &a, hipStream_t s);
 
 }  // namespace gx

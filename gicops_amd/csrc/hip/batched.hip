@@ -236,22 +236,24 @@ __global__ void __launch_bounds__(kPlanThreads) plan_slices_kernel(PlanArgs a) {
     const uint32_t lo = (uint32_t)a.pair_lo[pi];
     const uint32_t end_id = (uint32_t)(a.pair_hi[pi] + 1) * speclen;
     const uint32_t T = a.pair_pack[pi] * (uint32_t)a.tile_words;  // psids per tile
-    const uint32_t stride = tiles + 1;
     const uint64_t base = (a.pos_off[s] - a.pos_off[a.chunk_s0]) + a.pair_pos_off[pi];
     const int32_t *lq = a.lq + a.qidx[s];
     unsigned long long bins = 0;
+    // tile-major rows: row t holds boundary t of every slice, so that the
+    // search kernel's item for tile t reads rows t and t+1 coalesced
+    uint32_t *out = a.pos + base;
     for (int j = tid; j < nsl; j += kPlanThreads) {
         const int k = j / nb5;
         const uint32_t bin = (uint32_t)(lq[k] - dF + (j - k * nb5));
         const uint32_t st = gptr(bk.bA)[bin], en = gptr(bk.bA)[bin + 1];
-        uint32_t *out = a.pos + base + (uint64_t)j * stride;
         // bins of at most whole_bin ions are streamed whole by the search
-        // kernel (flagged start, every tile filters by id); larger ones get
-        // boundary t = first ion of the slice with id >= (lo + t*T)*speclen,
-        // the last one being the window end
+        // kernel: a flagged start in rows 0..tiles-1 and the end in the last
+        // row (every tile filters by id); larger ones get boundary t = first
+        // ion of the slice with id >= (lo + t*T)*speclen, the last one being
+        // the window end
         if (en - st <= a.whole_bin) {
-            out[0] = st | kWholeFlag;
-            out[1] = en;
+            for (uint32_t t = 0; t < tiles; t++) out[(uint64_t)t * nsl + j] = st | kWholeFlag;
+            out[(uint64_t)tiles * nsl + j] = en;
             bins++;
             continue;
         }
@@ -259,7 +261,7 @@ __global__ void __launch_bounds__(kPlanThreads) plan_slices_kernel(PlanArgs a) {
         for (uint32_t t = 0; t <= tiles; t++) {
             const uint32_t id = (t == tiles) ? end_id : (lo + t * T) * speclen;
             if (en > cur) cur = lower_bound_sampled(bk, cur, en, id);
-            out[t] = cur;
+            out[(uint64_t)t * nsl + j] = cur;
         }
         bins++;
     }
@@ -424,7 +426,7 @@ __global__ void __launch_bounds__(THREADS, WAVES_PER_SIMD) search_spectra_kernel
     for (int w = tid; w < GX_HIST_BINS; w += THREADS) L.hist[w] = 0;
     L.cmask[tid] = 0;
     for (int b = tid; b < a.n_buckets; b += THREADS) L.b_speclen[b] = a.buckets[b].speclen;
-    const uint32_t nspec = a.chunk_s1 - a.chunk_s0;
+    const uint32_t nunits = a.n_units;
     if (tid == 0) {
         L.next = atomicAdd(a.work_counter, 1u);
         L.n_cand = 0;
@@ -434,8 +436,10 @@ __global__ void __launch_bounds__(THREADS, WAVES_PER_SIMD) search_spectra_kernel
     for (;;) {
         lds_barrier();
         const uint32_t ticket = L.next;
-        if (ticket >= nspec) break;
-        const uint32_t s = a.order[a.chunk_s0 + ticket];
+        if (ticket >= nunits) break;
+        const uint32_t s = a.unit_s[a.unit0 + ticket];
+        const uint32_t br = a.unit_br[a.unit0 + ticket];
+        const uint32_t ub0 = br & 0xffu, ub1 = (br >> 8) & 0xffu, slot = br >> 16;
         lds_barrier();
         // fetch the next ticket now; its latency overlaps this spectrum
         if (tid == 0) L.next = atomicAdd(a.work_counter, 1u);
@@ -462,7 +466,8 @@ __global__ void __launch_bounds__(THREADS, WAVES_PER_SIMD) search_spectra_kernel
             w_lo = a.pair_lo[pi];
             w_hi = a.pair_hi[pi];
         }
-        for (unsigned long long todo = __ballot(w_tiles > 0); todo; todo &= todo - 1) {
+        for (unsigned long long todo = __ballot(w_tiles > 0 && (uint32_t)lane >= ub0 && (uint32_t)lane < ub1); todo;
+             todo &= todo - 1) {
             const uint32_t b = (uint32_t)__builtin_ctzll(todo);
             const uint32_t tiles = (uint32_t)__builtin_amdgcn_readlane((int)w_tiles, (int)b);
             const int32_t lo = __builtin_amdgcn_readlane(w_lo, (int)b), hi = __builtin_amdgcn_readlane(w_hi, (int)b);
@@ -475,7 +480,6 @@ __global__ void __launch_bounds__(THREADS, WAVES_PER_SIMD) search_spectra_kernel
             const uint32_t T = ((uint32_t)TILE_WORDS / (uint32_t)a.split) << lp;
             const DevBucket bk = a.buckets[b];
             const uint32_t speclen = bk.speclen;
-            const uint32_t stride = tiles + 1;
             const uint32_t *pos = a.pos + pos_s + poff;
             const auto ia = gptr(bk.iA);
             const uint32_t dm = bk.div_m, dsh = bk.div_s;
@@ -498,13 +502,15 @@ __global__ void __launch_bounds__(THREADS, WAVES_PER_SIMD) search_spectra_kernel
                     static_assert(SL == THREADS, "one slice per thread");
                     uint32_t at = 0, len = 0;
                     if (tid < ns) {
-                        // all four words at once (one round trip): a small bin is
-                        // [start | flag, end), a larger one has tile boundaries
-                        const uint32_t *ps = pos + (uint64_t)(c0 + tid) * stride;
-                        const uint32_t p0 = ps[0], p1 = ps[1], pt = ps[t], pt1 = ps[t + 1];
-                        if (p0 & kWholeFlag) {  // small bin: the whole bin, filtered by id below
-                            at = p0 & ~kWholeFlag;
-                            len = p1 - at;
+                        // rows t, t+1 and the last, at once (one round trip): a small
+                        // bin is [start | flag, end in the last row), a larger one has
+                        // tile boundaries in rows t and t+1
+                        const uint32_t j = (uint32_t)(c0 + tid);
+                        const uint32_t pt = pos[(uint64_t)t * nsl + j], pt1 = pos[(uint64_t)(t + 1) * nsl + j];
+                        const uint32_t pe = pos[(uint64_t)tiles * nsl + j];
+                        if (pt & kWholeFlag) {  // small bin: the whole bin, filtered by id below
+                            at = pt & ~kWholeFlag;
+                            len = pe - at;
                         } else {
                             at = pt;
                             len = pt1 - at;
@@ -652,8 +658,10 @@ __global__ void __launch_bounds__(THREADS, WAVES_PER_SIMD) search_spectra_kernel
             L.w_cps[wave] = bcps;
         }
         lds_barrier();
+        const bool whole = slot == kWholeUnit;
+        uint32_t *hout = whole ? a.hist + (size_t)s * GX_HIST_BINS : a.part_hist + (size_t)slot * GX_HIST_BINS;
         for (int w = tid; w < GX_HIST_BINS; w += THREADS) {
-            a.hist[(size_t)s * GX_HIST_BINS + w] = L.hist[w];
+            hout[w] = L.hist[w];
             L.hist[w] = 0;
         }
         if (tid == 0) {
@@ -663,10 +671,17 @@ __global__ void __launch_bounds__(THREADS, WAVES_PER_SIMD) search_spectra_kernel
                 cps += L.w_cps[w];
                 merge_best(key, eq, shp, L.w_key[w], L.w_eq[w], L.w_shp[w]);
             }
-            a.best_key[s] = key;
-            a.best_eq[s] = eq;
-            a.best_shp[s] = shp;
-            a.cpsms[s] = cps;
+            if (whole) {
+                a.best_key[s] = key;
+                a.best_eq[s] = eq;
+                a.best_shp[s] = shp;
+                a.cpsms[s] = cps;
+            } else {
+                a.part_key[slot] = key;
+                a.part_eq[slot] = eq;
+                a.part_shp[slot] = shp;
+                a.part_cps[slot] = cps;
+            }
             my_cand += cps;
         }
     }
@@ -678,6 +693,32 @@ __global__ void __launch_bounds__(THREADS, WAVES_PER_SIMD) search_spectra_kernel
         stat_add(a.stats, ST_IONS, my_scanned);
         stat_add(a.stats, ST_SCANNED, my_scanned);
         stat_add(a.stats, ST_ERRORS, my_errors);
+    }
+}
+
+// one wave per heavy spectrum of the chunk: its parts' histograms and
+// candidate counts add up, and their top hits merge with the same rule as
+// within a block (the parts are disjoint bucket ranges of one spectrum)
+__global__ void __launch_bounds__(64) fold_kernel(FoldArgs a) {
+    const int lane = threadIdx.x;
+    const uint32_t i = blockIdx.x;
+    const uint32_t s = a.f_s[i], p0 = a.f_slot0[i], n = a.f_n[i];
+    for (int w = lane; w < GX_HIST_BINS; w += 64) {
+        uint32_t h = 0;
+        for (uint32_t k = 0; k < n; k++) h += a.part_hist[(size_t)(p0 + k) * GX_HIST_BINS + w];
+        a.hist[(size_t)s * GX_HIST_BINS + w] = h;
+    }
+    if (lane == 0) {
+        unsigned long long key = 0;
+        uint32_t eq = 0, shp = 0, cps = 0;
+        for (uint32_t k = 0; k < n; k++) {
+            cps += a.part_cps[p0 + k];
+            merge_best(key, eq, shp, a.part_key[p0 + k], a.part_eq[p0 + k], a.part_shp[p0 + k]);
+        }
+        a.best_key[s] = key;
+        a.best_eq[s] = eq;
+        a.best_shp[s] = shp;
+        a.cpsms[s] = cps;
     }
 }
 
@@ -796,6 +837,12 @@ hipError_t launch_search_spectra(int config, const ItemArgs &a, int n_blocks, hi
         case 2: return launch_spectra<256, 4096, 256, 3>(a, n_blocks, s);
         default: return launch_spectra<768, 28672, 768, 3>(a, n_blocks, s);
     }
+}
+
+hipError_t launch_fold(const FoldArgs &a, hipStream_t s) {
+    if (a.n_folds == 0) return hipSuccess;
+    hipLaunchKernelGGL(fold_kernel, dim3(a.n_folds), dim3(64), 0, s, a);
+    return hipGetLastError();
 }
 
 hipError_t launch_evalue(const EvalArgs &a, hipStream_t s) {

@@ -323,7 +323,8 @@ extern "C" void gx_queries_free(gx_dqueries *d) {
                   d->hist,      d->lq,         d->li,            d->n_legal,      d->pair_lo,  d->pair_hi,
                   d->pair_tiles, d->pair_pack, d->pair_pos_off, d->n_items,     d->item_off, d->n_pos,
                   d->pos_off,   d->cpsms,      d->counters,      d->best_key,     d->best_eq,  d->best_shp,
-                  d->pos,       d->order,      d->porder,           d->l_feat,       d->l_h1,     d->l_h2,
+                  d->pos,       d->unit_s,     d->unit_br,       d->fold_s,       d->fold_p0,  d->fold_n,
+                  d->part_key,  d->part_eq,    d->part_shp,      d->part_cps,     d->part_hist, d->porder,           d->l_feat,       d->l_h1,     d->l_h2,
                   d->l_anchor,  d->l_tgt,      d->l_loge,     d->l_out,         d->l_valid,      d->l_sparse, d->l_route};
     for (void *p : ps)
         if (p) (void)hipFree(p);
@@ -425,13 +426,21 @@ int ensure_batched(gx_dqueries *q, const gx_dindex *idx) {
     q->nb = idx->n_buckets;
     if (dalloc(&q->lq, np) || dalloc(&q->li, np) || dalloc(&q->n_legal, ns) || dalloc(&q->pair_lo, npairs) ||
         dalloc(&q->pair_hi, npairs) || dalloc(&q->pair_tiles, npairs) || dalloc(&q->pair_pack, npairs) ||
-        dalloc(&q->pair_pos_off, npairs) || dalloc(&q->n_items, ns) || dalloc(&q->best_key, ns) || dalloc(&q->order, ns) || dalloc(&q->porder, ns) ||
+        dalloc(&q->pair_pos_off, npairs) || dalloc(&q->n_items, ns) || dalloc(&q->best_key, ns) || dalloc(&q->porder, ns) ||
         dalloc(&q->best_eq, ns) || dalloc(&q->best_shp, ns) ||
         dalloc(&q->item_off, ns + 1) || dalloc(&q->n_pos, ns) || dalloc(&q->pos_off, ns + 1) ||
         dalloc(&q->cpsms, ns) || (!q->hist && dalloc(&q->hist, ns * GX_HIST_BINS)))
         return GX_ERR_BAD_MEM_ALLOC;
     q->h_pos_off.resize(ns + 1);
     return GX_SUCCESS;
+}
+
+// replace a device array by a new one of n elements (contents not kept)
+template <class T>
+int realloc_dev(T **p, size_t n) {
+    if (*p) (void)hipFree(*p);
+    *p = nullptr;
+    return dalloc(p, n);
 }
 
 template <class T>
@@ -521,7 +530,13 @@ void fill_item_args(Batched &B, gx_ctx *c, const gx_dindex *idx, gx_dqueries *q,
     ia.pair_pos_off = q->pair_pos_off;
     ia.pos_off = q->pos_off;
     ia.pos = q->pos;
-    ia.order = q->order;
+    ia.unit_s = q->unit_s;
+    ia.unit_br = q->unit_br;
+    ia.part_key = q->part_key;
+    ia.part_eq = q->part_eq;
+    ia.part_shp = q->part_shp;
+    ia.part_cps = q->part_cps;
+    ia.part_hist = q->part_hist;
     ia.cand_scratch = c->d_cand;
     ia.tile_entries_max = B.cfg.tile_words * 4;
     ia.split = B.split;
@@ -575,9 +590,28 @@ hipError_t enqueue_chunks(gx_ctx *c, gx_dqueries *q, Batched &B, hipStream_t sm,
         if (events && (e = hipEventRecord(c->ev_chunk[2 * k + 1], sm))) return e;
         B.ia.chunk_s0 = q->bounds[k];
         B.ia.chunk_s1 = q->bounds[k + 1];
+        B.ia.unit0 = q->unit_bounds[k];
+        B.ia.n_units = q->unit_bounds[k + 1] - q->unit_bounds[k];
         B.ia.work_counter = q->counters + k;
-        // every spectrum's results are written by the search kernel
+        // whole spectra are written by the search kernel, the parts of heavy
+        // ones by the fold that follows it
         if ((e = launch_search_spectra(B.cfg_id, B.ia, B.n_blocks, sm))) return e;
+        FoldArgs fa{};
+        fa.f_s = q->fold_s + q->fold_bounds[k];
+        fa.f_slot0 = q->fold_p0 + q->fold_bounds[k];
+        fa.f_n = q->fold_n + q->fold_bounds[k];
+        fa.n_folds = q->fold_bounds[k + 1] - q->fold_bounds[k];
+        fa.part_key = q->part_key;
+        fa.part_eq = q->part_eq;
+        fa.part_shp = q->part_shp;
+        fa.part_cps = q->part_cps;
+        fa.part_hist = q->part_hist;
+        fa.hist = q->hist;
+        fa.cpsms = q->cpsms;
+        fa.best_eq = q->best_eq;
+        fa.best_shp = q->best_shp;
+        fa.best_key = q->best_key;
+        if ((e = launch_fold(fa, sm))) return e;
     }
     if (events && (e = hipEventRecord(c->ev2, sm))) return e;
     if (B.learned) return enqueue_learned_evalue(c, q, B.ea, B.conf, sm);
@@ -649,22 +683,143 @@ int search_batched(gx_ctx *c, const gx_dindex *idx, gx_dqueries *q, const gx_sea
         }
     }
     const size_t n_chunks = q->bounds.size() - 1;
-    // Spectra are handed out largest first within each chunk (the slice
-    // positions a spectrum needs stand in for its work), so that the chunk
-    // ends on small spectra: a counting sort by the bit length of the work.
-    q->h_order.resize(ns);
-    for (size_t k = 0; k < n_chunks; k++) {
-        const uint32_t s0 = q->bounds[k], s1 = q->bounds[k + 1];
-        uint32_t cnt[66] = {};
-        auto cls = [&](uint32_t s) { return 64 - __builtin_clzll(q->h_pos_off[s + 1] - q->h_pos_off[s] + 1); };
-        for (uint32_t s = s0; s < s1; s++) cnt[65 - cls(s)]++;
-        uint32_t run = s0;
-        for (int c = 0; c < 66; c++) {
-            const uint32_t n = cnt[c];
-            cnt[c] = run;
-            run += n;
+    // Work units. The slice positions a spectrum needs stand in for its work.
+    // A spectrum heavier than half of one block's fair share of its chunk is
+    // cut into parts of contiguous buckets, each at most that heavy, searched
+    // by different blocks into slots of the part scratch and folded after the
+    // chunk's search. Units are handed out largest first (a counting sort by
+    // the bit length of the work), so that a chunk ends on small units.
+    {
+        const uint32_t nb = (uint32_t)idx->n_buckets;
+        std::vector<uint64_t> cut(n_chunks);
+        bool any_heavy = false;
+        for (size_t k = 0; k < n_chunks; k++) {
+            const uint32_t s0 = q->bounds[k], s1 = q->bounds[k + 1];
+            cut[k] = std::max<uint64_t>(1, (q->h_pos_off[s1] - q->h_pos_off[s0]) / (2 * (uint64_t)B.n_blocks));
+            for (uint32_t s = s0; s < s1 && !any_heavy; s++)
+                any_heavy = q->h_pos_off[s + 1] - q->h_pos_off[s] > cut[k];
         }
-        for (uint32_t s = s0; s < s1; s++) q->h_order[cnt[65 - cls(s)]++] = s;
+        if (any_heavy && nb > 1) {
+            q->h_pair_off.resize((size_t)ns * nb);
+            GX_HIP(hipMemcpyAsync(q->h_pair_off.data(), q->pair_pos_off, sizeof(uint32_t) * (size_t)ns * nb,
+                                  hipMemcpyDeviceToHost, sm));
+            GX_HIP(hipStreamSynchronize(sm));
+        }
+        q->h_unit_s.clear();
+        q->h_unit_br.clear();
+        q->h_fold_s.clear();
+        q->h_fold_p0.clear();
+        q->h_fold_n.clear();
+        q->unit_bounds.assign(1, 0);
+        q->fold_bounds.assign(1, 0);
+        uint32_t max_parts = 0, parts_total = 0;
+        std::vector<uint32_t> us, ubr;
+        std::vector<uint64_t> uw;
+        std::vector<uint32_t> pb;  // part boundaries (bucket indices) of one spectrum
+        for (size_t k = 0; k < n_chunks; k++) {
+            const uint32_t s0 = q->bounds[k], s1 = q->bounds[k + 1];
+            us.clear();
+            ubr.clear();
+            uw.clear();
+            uint32_t slot = 0;
+            for (uint32_t s = s0; s < s1; s++) {
+                const uint64_t w = q->h_pos_off[s + 1] - q->h_pos_off[s];
+                pb.clear();
+                if (w > cut[k] && nb > 1 && any_heavy) {
+                    const uint32_t *po = q->h_pair_off.data() + (size_t)s * nb;
+                    uint64_t acc = 0;
+                    pb.push_back(0);
+                    for (uint32_t b = 0; b < nb; b++) {
+                        const uint64_t wb = (b + 1 < nb ? (uint64_t)po[b + 1] : w) - po[b];
+                        if (acc > 0 && acc + wb > cut[k]) {
+                            pb.push_back(b);
+                            acc = 0;
+                        }
+                        acc += wb;
+                    }
+                    pb.push_back(nb);
+                }
+                const uint32_t np = pb.empty() ? 0u : (uint32_t)pb.size() - 1;
+                if (np >= 2 && slot + np < kWholeUnit) {
+                    const uint32_t *po = q->h_pair_off.data() + (size_t)s * nb;
+                    q->h_fold_s.push_back(s);
+                    q->h_fold_p0.push_back(slot);
+                    q->h_fold_n.push_back(np);
+                    for (uint32_t i = 0; i < np; i++) {
+                        const uint32_t b0 = pb[i], b1 = pb[i + 1];
+                        const uint64_t e = b1 < nb ? (uint64_t)po[b1] : w;
+                        us.push_back(s);
+                        ubr.push_back(b0 | (b1 << 8) | (slot << 16));
+                        uw.push_back(e - po[b0]);
+                        slot++;
+                    }
+                } else {
+                    us.push_back(s);
+                    ubr.push_back(0u | (nb << 8) | (kWholeUnit << 16));
+                    uw.push_back(w);
+                }
+            }
+            const uint32_t nu = (uint32_t)us.size();
+            uint32_t cnt[66] = {};
+            auto cls = [&](uint32_t i) { return 64 - __builtin_clzll(uw[i] + 1); };
+            for (uint32_t i = 0; i < nu; i++) cnt[65 - cls(i)]++;
+            uint32_t run = 0;
+            for (int c = 0; c < 66; c++) {
+                const uint32_t n = cnt[c];
+                cnt[c] = run;
+                run += n;
+            }
+            const size_t base = q->h_unit_s.size();
+            q->h_unit_s.resize(base + nu);
+            q->h_unit_br.resize(base + nu);
+            for (uint32_t i = 0; i < nu; i++) {
+                const uint32_t r = cnt[65 - cls(i)]++;
+                q->h_unit_s[base + r] = us[i];
+                q->h_unit_br[base + r] = ubr[i];
+            }
+            q->unit_bounds.push_back((uint32_t)q->h_unit_s.size());
+            q->fold_bounds.push_back((uint32_t)q->h_fold_s.size());
+            max_parts = std::max(max_parts, slot);
+            parts_total += slot;
+        }
+        q->n_parts_total = parts_total;
+        const size_t nu = q->h_unit_s.size(), nf = std::max<size_t>(1, q->h_fold_s.size());
+        const size_t npart = std::max<uint32_t>(1, max_parts);
+        if (nu > q->unit_cap || !q->unit_s) {
+            q->buf_gen++;
+            const size_t want = nu + nu / 8 + 64;
+            q->unit_cap = 0;
+            if (realloc_dev(&q->unit_s, want) || realloc_dev(&q->unit_br, want)) return GX_ERR_BAD_MEM_ALLOC;
+            q->unit_cap = want;
+        }
+        if (nf > q->fold_cap || !q->fold_s) {
+            q->buf_gen++;
+            const size_t want = nf + nf / 8 + 64;
+            q->fold_cap = 0;
+            if (realloc_dev(&q->fold_s, want) || realloc_dev(&q->fold_p0, want) || realloc_dev(&q->fold_n, want))
+                return GX_ERR_BAD_MEM_ALLOC;
+            q->fold_cap = want;
+        }
+        if (npart > q->part_cap || !q->part_key) {
+            q->buf_gen++;
+            const size_t want = npart + npart / 8 + 16;
+            q->part_cap = 0;
+            if (realloc_dev(&q->part_key, want) || realloc_dev(&q->part_eq, want) ||
+                realloc_dev(&q->part_shp, want) || realloc_dev(&q->part_cps, want) ||
+                realloc_dev(&q->part_hist, want * GX_HIST_BINS))
+                return GX_ERR_BAD_MEM_ALLOC;
+            q->part_cap = want;
+        }
+        if (nu) {
+            GX_HIP(hipMemcpyAsync(q->unit_s, q->h_unit_s.data(), 4 * nu, hipMemcpyHostToDevice, sm));
+            GX_HIP(hipMemcpyAsync(q->unit_br, q->h_unit_br.data(), 4 * nu, hipMemcpyHostToDevice, sm));
+        }
+        if (!q->h_fold_s.empty()) {
+            const size_t n = q->h_fold_s.size();
+            GX_HIP(hipMemcpyAsync(q->fold_s, q->h_fold_s.data(), 4 * n, hipMemcpyHostToDevice, sm));
+            GX_HIP(hipMemcpyAsync(q->fold_p0, q->h_fold_p0.data(), 4 * n, hipMemcpyHostToDevice, sm));
+            GX_HIP(hipMemcpyAsync(q->fold_n, q->h_fold_n.data(), 4 * n, hipMemcpyHostToDevice, sm));
+        }
     }
     // the plan visits each chunk's spectra by precursor mass (1 Da classes):
     // neighbouring blocks then share windows, and their boundary searches
@@ -682,10 +837,7 @@ int search_batched(gx_ctx *c, const gx_dindex *idx, gx_dqueries *q, const gx_sea
         for (uint32_t c = 0; c < kCls; c++) cnt[c + 1] += cnt[c];
         for (uint32_t s = s0; s < s1; s++) q->h_porder[s0 + cnt[cls(s)]++] = s;
     }
-    if (ns) {
-        GX_HIP(hipMemcpyAsync(q->order, q->h_order.data(), sizeof(uint32_t) * ns, hipMemcpyHostToDevice, sm));
-        GX_HIP(hipMemcpyAsync(q->porder, q->h_porder.data(), sizeof(uint32_t) * ns, hipMemcpyHostToDevice, sm));
-    }
+    if (ns) GX_HIP(hipMemcpyAsync(q->porder, q->h_porder.data(), sizeof(uint32_t) * ns, hipMemcpyHostToDevice, sm));
     if (grow(&q->pos, q->pos_cap, max_chunk, q->buf_gen)) return GX_ERR_BAD_MEM_ALLOC;
     if (!q->counters || n_chunks > q->counters_n) {
         q->buf_gen++;

@@ -86,9 +86,20 @@ struct gx_dqueries {
     size_t counters_n = 0;
     unsigned long long *best_key = nullptr;  // per-spectrum top hit of the search kernel
     uint32_t *best_eq = nullptr, *best_shp = nullptr;
-    uint32_t *order = nullptr;             // spectra of each chunk, largest first (search)
     uint32_t *porder = nullptr;            // spectra of each chunk by precursor mass (plan)
-    std::vector<uint32_t> h_order, h_porder;
+    std::vector<uint32_t> h_porder;
+    // work units of the search kernel (spectra, or bucket-range parts of heavy
+    // spectra), largest first within each chunk, and the folds of the parts
+    uint32_t *unit_s = nullptr, *unit_br = nullptr;
+    size_t unit_cap = 0;
+    uint32_t *fold_s = nullptr, *fold_p0 = nullptr, *fold_n = nullptr;
+    size_t fold_cap = 0;
+    unsigned long long *part_key = nullptr;
+    uint32_t *part_eq = nullptr, *part_shp = nullptr, *part_cps = nullptr, *part_hist = nullptr;
+    size_t part_cap = 0;
+    std::vector<uint32_t> h_unit_s, h_unit_br, unit_bounds, h_fold_s, h_fold_p0, h_fold_n, fold_bounds;
+    std::vector<uint32_t> h_pair_off;  // per-pair slice-position offsets (heavy-spectrum splitting)
+    uint32_t n_parts_total = 0;
     std::vector<float> h_prec;             // precursor masses (host copy)
     uint32_t *pos = nullptr;
     size_t pos_cap = 0;  // words
