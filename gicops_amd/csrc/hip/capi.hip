@@ -456,7 +456,7 @@ int grow(T **p, size_t &cap, size_t n, uint64_t &gen) {
     return st;
 }
 
-constexpr uint64_t kPosChunkWords = 1ull << 29;  // 2 GB of slice positions per planning chunk
+constexpr uint64_t kPosChunkWords = 1ull << 31;  // 8 GB of slice positions per planning chunk
 
 struct Batched {
     int cfg_id;
@@ -684,8 +684,8 @@ int search_batched(gx_ctx *c, const gx_dindex *idx, gx_dqueries *q, const gx_sea
     }
     const size_t n_chunks = q->bounds.size() - 1;
     // Work units. The slice positions a spectrum needs stand in for its work.
-    // A spectrum heavier than half of one block's fair share of its chunk is
-    // cut into parts of contiguous buckets, each at most that heavy, searched
+    // A spectrum heavier than an eighth of one block's fair share of its chunk
+    // is cut into parts of contiguous buckets, each at most that heavy, searched
     // by different blocks into slots of the part scratch and folded after the
     // chunk's search. Units are handed out largest first (a counting sort by
     // the bit length of the work), so that a chunk ends on small units.
@@ -695,7 +695,7 @@ int search_batched(gx_ctx *c, const gx_dindex *idx, gx_dqueries *q, const gx_sea
         bool any_heavy = false;
         for (size_t k = 0; k < n_chunks; k++) {
             const uint32_t s0 = q->bounds[k], s1 = q->bounds[k + 1];
-            cut[k] = std::max<uint64_t>(1, (q->h_pos_off[s1] - q->h_pos_off[s0]) / (2 * (uint64_t)B.n_blocks));
+            cut[k] = std::max<uint64_t>(1, (q->h_pos_off[s1] - q->h_pos_off[s0]) / (8 * (uint64_t)B.n_blocks));
             for (uint32_t s = s0; s < s1 && !any_heavy; s++)
                 any_heavy = q->h_pos_off[s + 1] - q->h_pos_off[s] > cut[k];
         }
