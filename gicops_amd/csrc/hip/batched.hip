@@ -466,13 +466,27 @@ __global__ void __launch_bounds__(THREADS, WAVES_PER_SIMD) search_spectra_kernel
             w_lo = a.pair_lo[pi];
             w_hi = a.pair_hi[pi];
         }
-        for (unsigned long long todo = __ballot(w_tiles > 0 && (uint32_t)lane >= ub0 && (uint32_t)lane < ub1); todo;
-             todo &= todo - 1) {
+        // the slice words of the next (bucket, tile, slice chunk) step are
+        // loaded one step ahead, so that their latency overlaps this step
+        uint32_t nx_pt = 0, nx_pt1 = 0, nx_pe = 0;
+        auto fetch = [&](uint32_t fb, uint32_t ft, int fc0) {
+            const uint32_t ftiles = (uint32_t)__builtin_amdgcn_readlane((int)w_tiles, (int)fb);
+            const uint32_t fpoff = (uint32_t)__builtin_amdgcn_readlane((int)w_poff, (int)fb);
+            const uint32_t *fpos = a.pos + pos_s + fpoff;
+            if (tid < min(SL, nsl - fc0)) {
+                const uint32_t j = (uint32_t)(fc0 + tid);
+                nx_pt = fpos[(uint64_t)ft * nsl + j];
+                nx_pt1 = fpos[(uint64_t)(ft + 1) * nsl + j];
+                nx_pe = fpos[(uint64_t)ftiles * nsl + j];
+            }
+        };
+        const unsigned long long todo0 = __ballot(w_tiles > 0 && (uint32_t)lane >= ub0 && (uint32_t)lane < ub1);
+        if (todo0) fetch((uint32_t)__builtin_ctzll(todo0), 0, 0);
+        for (unsigned long long todo = todo0; todo; todo &= todo - 1) {
             const uint32_t b = (uint32_t)__builtin_ctzll(todo);
             const uint32_t tiles = (uint32_t)__builtin_amdgcn_readlane((int)w_tiles, (int)b);
             const int32_t lo = __builtin_amdgcn_readlane(w_lo, (int)b), hi = __builtin_amdgcn_readlane(w_hi, (int)b);
             const uint32_t pack = (uint32_t)__builtin_amdgcn_readlane((int)w_pack, (int)b);
-            const uint32_t poff = (uint32_t)__builtin_amdgcn_readlane((int)w_poff, (int)b);
             const uint32_t lp = pack == 4 ? 2u : (pack == 2 ? 1u : 0u);  // log2 entries per word
             const uint32_t fbits = 32u >> lp;                             // counter width
             const uint32_t fmask = fbits == 32 ? 0xffffffffu : ((1u << fbits) - 1u);
@@ -480,7 +494,6 @@ __global__ void __launch_bounds__(THREADS, WAVES_PER_SIMD) search_spectra_kernel
             const uint32_t T = ((uint32_t)TILE_WORDS / (uint32_t)a.split) << lp;
             const DevBucket bk = a.buckets[b];
             const uint32_t speclen = bk.speclen;
-            const uint32_t *pos = a.pos + pos_s + poff;
             const auto ia = gptr(bk.iA);
             const uint32_t dm = bk.div_m, dsh = bk.div_s;
             const uint32_t btag = b << 26;
@@ -501,13 +514,20 @@ __global__ void __launch_bounds__(THREADS, WAVES_PER_SIMD) search_spectra_kernel
                     // slice ranges of this tile; non-empty ones compacted in order
                     static_assert(SL == THREADS, "one slice per thread");
                     uint32_t at = 0, len = 0;
+                    // this step's words (rows t, t+1 and the last of its slices),
+                    // then the next step's load
+                    const uint32_t pt = nx_pt, pt1 = nx_pt1, pe = nx_pe;
+                    if (c0 + SL < nsl) {
+                        fetch(b, t, c0 + SL);
+                    } else if (t + 1 < tiles) {
+                        fetch(b, t + 1, 0);
+                    } else {
+                        const unsigned long long rest = todo & (todo - 1);
+                        if (rest) fetch((uint32_t)__builtin_ctzll(rest), 0, 0);
+                    }
                     if (tid < ns) {
-                        // rows t, t+1 and the last, at once (one round trip): a small
-                        // bin is [start | flag, end in the last row), a larger one has
-                        // tile boundaries in rows t and t+1
-                        const uint32_t j = (uint32_t)(c0 + tid);
-                        const uint32_t pt = pos[(uint64_t)t * nsl + j], pt1 = pos[(uint64_t)(t + 1) * nsl + j];
-                        const uint32_t pe = pos[(uint64_t)tiles * nsl + j];
+                        // a small bin is [start | flag, end in the last row), a larger
+                        // one has tile boundaries in rows t and t+1
                         if (pt & kWholeFlag) {  // small bin: the whole bin, filtered by id below
                             at = pt & ~kWholeFlag;
                             len = pe - at;
