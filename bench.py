@@ -68,9 +68,10 @@ def traffic_from_profiles(workload: str, config: int, whole_bin: int):
         if ":" in line and not line.startswith("#"):
             k, v = line.split(":", 1)
             vals[k.strip()] = v.strip()
-    if int(vals.get("launch_config", -1)) != config or int(vals.get("whole_bin", -1)) != whole_bin:
+    if int(vals.get("launch_config", -1)) != config:
         return None
-    return float(vals["hbm_bytes_per_launch"]) if "hbm_bytes_per_launch" in vals else None
+    key = f"whole_bin_{whole_bin}_hbm_bytes_per_launch"
+    return float(vals[key]) if key in vals else None
 
 
 def cpu_baseline(index, batch, params_ffi, budget_s: float):
