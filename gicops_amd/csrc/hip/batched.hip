@@ -213,6 +213,67 @@ __device__ __forceinline__ uint32_t lower_bound_sampled(const DevBucket &bk, uin
     return lo + lower_bound_u32(gptr(bk.iA) + lo, hi - lo, x);
 }
 
+// the same for two keys of one bin at once: the two searches' loads are
+// independent, so each step waits for one latency instead of two
+__device__ __forceinline__ void lower_bound_sampled2(const DevBucket &bk, uint32_t st, uint32_t en, uint32_t x0,
+                                                     uint32_t x1, uint32_t &r0, uint32_t &r1) {
+    uint32_t lo0 = st, hi0 = en, lo1 = st, hi1 = en;
+    const uint32_t k0 = (st + 31u) >> 5;
+    if (en > 0 && k0 <= ((en - 1u) >> 5)) {
+        const uint32_t k1 = (en - 1u) >> 5;
+        const auto is = gptr(bk.iS);
+        uint32_t b0 = k0, b1 = k0, n0 = k1 - k0 + 1u, n1 = n0;
+        while (n0 > 0 || n1 > 0) {
+            const uint32_t h0 = n0 >> 1, h1 = n1 >> 1;
+            const uint32_t v0 = n0 > 0 ? is[b0 + h0] : 0u, v1 = n1 > 0 ? is[b1 + h1] : 0u;
+            if (n0 > 0) {
+                if (v0 < x0) {
+                    b0 += h0 + 1u;
+                    n0 -= h0 + 1u;
+                } else {
+                    n0 = h0;
+                }
+            }
+            if (n1 > 0) {
+                if (v1 < x1) {
+                    b1 += h1 + 1u;
+                    n1 -= h1 + 1u;
+                } else {
+                    n1 = h1;
+                }
+            }
+        }
+        if (b0 <= k1) hi0 = min(hi0, b0 << 5);
+        if (b0 > k0) lo0 = max(lo0, ((b0 - 1u) << 5) + 1u);
+        if (b1 <= k1) hi1 = min(hi1, b1 << 5);
+        if (b1 > k0) lo1 = max(lo1, ((b1 - 1u) << 5) + 1u);
+    }
+    const auto ia = gptr(bk.iA);
+    uint32_t n0 = hi0 - lo0, n1 = hi1 - lo1;
+    while (n0 > 0 || n1 > 0) {
+        const uint32_t h0 = n0 >> 1, h1 = n1 >> 1;
+        const uint32_t v0 = n0 > 0 ? ia[lo0 + h0] : 0u, v1 = n1 > 0 ? ia[lo1 + h1] : 0u;
+        if (n0 > 0) {
+            if (v0 < x0) {
+                lo0 += h0 + 1u;
+                n0 -= h0 + 1u;
+            } else {
+                n0 = h0;
+            }
+        }
+        if (n1 > 0) {
+            if (v1 < x1) {
+                lo1 += h1 + 1u;
+                n1 -= h1 + 1u;
+            } else {
+                n1 = h1;
+            }
+        }
+    }
+    r0 = lo0;
+    r1 = lo1;
+}
+
 // one block per (bucket, spectrum) of the chunk, bucket-major and by precursor
 // mass within a bucket: the blocks in flight read one bucket's bA and iA with
 // similar windows, which stay in L2
@@ -257,11 +318,21 @@ __global__ void __launch_bounds__(kPlanThreads) plan_slices_kernel(PlanArgs a) {
             bins++;
             continue;
         }
+        // boundaries two at a time, each pair searched from the previous one
         uint32_t cur = st;
-        for (uint32_t t = 0; t <= tiles; t++) {
-            const uint32_t id = (t == tiles) ? end_id : (lo + t * T) * speclen;
-            if (en > cur) cur = lower_bound_sampled(bk, cur, en, id);
-            out[(uint64_t)t * nsl + j] = cur;
+        for (uint32_t t = 0; t <= tiles; t += 2) {
+            const uint32_t id0 = (t == tiles) ? end_id : (lo + t * T) * speclen;
+            if (t + 1 <= tiles) {
+                const uint32_t id1 = (t + 1 == tiles) ? end_id : (lo + (t + 1) * T) * speclen;
+                uint32_t c0 = cur, c1 = cur;
+                if (en > cur) lower_bound_sampled2(bk, cur, en, id0, id1, c0, c1);
+                out[(uint64_t)t * nsl + j] = c0;
+                out[(uint64_t)(t + 1) * nsl + j] = c1;
+                cur = c1;
+            } else {
+                if (en > cur) cur = lower_bound_sampled(bk, cur, en, id0);
+                out[(uint64_t)t * nsl + j] = cur;
+            }
         }
         bins++;
     }
