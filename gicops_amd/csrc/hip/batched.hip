@@ -26,6 +26,7 @@ namespace gx {
 namespace {
 
 constexpr int kPlanThreads = 256;
+constexpr int kSliceThreads = 128;  // plan_slices block (small blocks keep more of them in flight)
 constexpr int kPeakCap = kMaxPeaksLds;  // 2048 legal peaks per spectrum
 constexpr int kStatSpread = kStatRows;
 constexpr int kUnroll = 4;              // pieces in flight per wave
@@ -277,7 +278,7 @@ __device__ __forceinline__ void lower_bound_sampled2(const DevBucket &bk, uint32
 // one block per (bucket, spectrum) of the chunk, bucket-major and by precursor
 // mass within a bucket: the blocks in flight read one bucket's bA and iA with
 // similar windows, which stay in L2
-__global__ void __launch_bounds__(kPlanThreads) plan_slices_kernel(PlanArgs a) {
+__global__ void __launch_bounds__(kSliceThreads) plan_slices_kernel(PlanArgs a) {
     __shared__ unsigned long long s_st[2];
     const int tid = threadIdx.x, lane = tid & 63;
     const uint32_t nb = (uint32_t)a.n_buckets;
@@ -303,7 +304,7 @@ __global__ void __launch_bounds__(kPlanThreads) plan_slices_kernel(PlanArgs a) {
     // tile-major rows: row t holds boundary t of every slice, so that the
     // search kernel's item for tile t reads rows t and t+1 coalesced
     uint32_t *out = a.pos + base;
-    for (int j = tid; j < nsl; j += kPlanThreads) {
+    for (int j = tid; j < nsl; j += kSliceThreads) {
         const int k = j / nb5;
         const uint32_t bin = (uint32_t)(lq[k] - dF + (j - k * nb5));
         const uint32_t st = gptr(bk.bA)[bin], en = gptr(bk.bA)[bin + 1];
@@ -917,7 +918,7 @@ hipError_t launch_scan(const PlanArgs &a, hipStream_t s) {
 hipError_t launch_plan_slices(const PlanArgs &a, hipStream_t s) {
     const uint64_t nblk = (uint64_t)(a.chunk_s1 - a.chunk_s0) * (uint64_t)a.n_buckets;
     if (nblk == 0) return hipSuccess;
-    hipLaunchKernelGGL(plan_slices_kernel, dim3((unsigned)nblk), dim3(kPlanThreads), 0, s, a);
+    hipLaunchKernelGGL(plan_slices_kernel, dim3((unsigned)nblk), dim3(kSliceThreads), 0, s, a);
     return hipGetLastError();
 }
 
