@@ -100,6 +100,38 @@ def test_uneven_batch_parity(open_index, searcher_open, eng):
     compare(g, c, hg, hc, "uneven")
 
 
+Z3_MODS = slm.DBConfig(n_peptides=20_000, seed=31, maxz=3,
+                       mods=[("M", 15.9949, 2), ("STY", 79.96633, 1), ("^", 42.0106, 1)], max_mods_per_pep=3)
+
+
+@pytest.fixture(scope="module")
+def z3_index():
+    return slm.SlmIndex(Z3_MODS)
+
+
+@pytest.fixture(scope="module")
+def searcher_z3(z3_index):
+    from gicops_amd.search import Searcher
+
+    s = Searcher(z3_index)
+    yield s
+    s.close()
+
+
+@pytest.mark.parametrize("eng", ENGINES, ids=ENGINE_IDS)
+def test_charge3_modified_index_parity(z3_index, searcher_z3, eng):
+    """fragment charges 1-3 (speclen 6(L-1)), three variable modification
+    types incl. the N-terminal one, dM mix"""
+    say(f"case: maxz 3, 3 modification types, dM mix, {eng}")
+    cfg = slm.SpecConfig(n_spectra=240, seed=35, peak_mode=F.GX_PEAKS_TOTAL_UNIFORM, peaks_min=50,
+                         peaks_max=300, charge_min=2, charge_max=4, qalen=256, dm_choices=(10.0, 100.0, 500.0))
+    batch = slm.generate_spectra(z3_index, cfg)
+    g, hg, st = searcher_z3.search(batch, SearchParams(dM=500.0, **eng), want_hist=True)
+    c, hc, cnt = O.search(z3_index, batch, O.default_params(dM=500.0), want_hist=True, want_counters=True)
+    check_stats(st, cnt, batch.n_spectra)
+    compare(g, c, hg, hc, "maxz 3")
+
+
 @pytest.mark.parametrize("eng", ENGINES, ids=ENGINE_IDS)
 def test_device_resident_path_matches_one_call(small_batch, searcher_small, eng):
     say(f"case: device-resident upload/search/download vs one-call search {eng}")
