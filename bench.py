@@ -313,6 +313,13 @@ def main() -> int:
     log(f"{value:.0f} spectra/s, kernel {avg_kms:.1f} ms, {achieved:.0f} GB/s algorithmic, "
         f"H={st['ions']} B={st['bins']} W={st['window']} cand={st['candidates']} reported={reported}")
 
+    measured = {}
+    try:
+        measured = {"sequential": round(s.read_bandwidth(0), 1), "random_1x128B": round(s.read_bandwidth(1), 1),
+                    "random_2x128B": round(s.read_bandwidth(2), 1)}
+        log(f"measured read bandwidth: {measured}")
+    except Exception as e:  # diagnostic only
+        log(f"read bandwidth not measured: {e}")
     cpu = None
     if not args.no_cpu:
         # a GPU-built index is searched on the CPU through a host copy of its arrays
@@ -366,6 +373,10 @@ def main() -> int:
             "algorithmic_bytes_rule": "4 B per window-restricted iA id (H)",
             "job_algorithmic_bytes": abytes,
             "job_algorithmic_gbs": round(abytes / (ms_step / 1000.0) / 1e9, 1),
+            # measured on this device (gx_measure_read_bandwidth), beside the published peak
+            "measured_read_gbs": measured,
+            "frac_of_measured_random_2x128B": (round(achieved / measured["random_2x128B"], 4)
+                                               if measured.get("random_2x128B") else None),
         },
         "cpu_baseline": cpu,
     }

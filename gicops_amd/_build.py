@@ -39,6 +39,7 @@ HIP_SOURCES = [
     CSRC / "hip" / "index_build.hip",
     CSRC / "hip" / "preprocess.hip",
     CSRC / "hip" / "partials.hip",
+    CSRC / "hip" / "bandwidth.hip",
 ]
 HIP_HEADERS = sorted((CSRC / "hip").glob("*.h")) + sorted(INCLUDE.glob("*.h"))
 

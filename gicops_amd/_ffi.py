@@ -281,6 +281,7 @@ def hip_lib() -> C.CDLL:
                                               C.POINTER(vp)]
         lib.gx_queries_shape.argtypes = [vp, C.POINTER(C.c_uint32), C.POINTER(C.c_uint64), C.POINTER(C.c_float)]
         lib.gx_queries_download.argtypes = [vp, vp, vp, vp]
+        lib.gx_measure_read_bandwidth.argtypes = [vp, C.c_int, C.POINTER(C.c_double)]
         lib.gx_partials_export.argtypes = [vp, vp, vp, vp, vp, vp, C.c_int]
         lib.gx_partials_merge.argtypes = [vp, C.c_int, C.POINTER(vp), C.POINTER(vp), C.c_uint32,
                                           C.POINTER(gx_search_params), C.c_int, vp, vp, C.POINTER(C.c_float)]

@@ -147,6 +147,13 @@ class Searcher:
                                             F.ptr(iA) if iA.size else None))
         return bA, iA
 
+    def read_bandwidth(self, segment_lines: int = 0) -> float:
+        """Measured read bandwidth (GB/s): sequential (0) or random segments of
+        1, 2, 4 or 8 consecutive 128-B lines (gx_measure_read_bandwidth)."""
+        g = C.c_double()
+        F.check("gx_measure_read_bandwidth", self._lib.gx_measure_read_bandwidth(self._ctx, segment_lines, C.byref(g)))
+        return g.value
+
     def host_index(self) -> "HostIndex":
         """A host copy of the device index: the host index's entries with the
         device's bA/iA downloaded (for CPU-side checks of a GPU-built index)."""

@@ -311,6 +311,13 @@ int gx_partials_merge(gx_ctx *ctx, int n_parts, const gx_partial *const *rec, co
                       uint32_t n_spectra, const gx_search_params *p, int on_device, gx_psm *psms,
                       uint32_t *merged_hist, float *device_ms);
 
+/* Measured read bandwidth of the context's device in GB/s (HIP events): a
+ * sequential read of a 4 GiB buffer (segment_lines = 0), or reads of random
+ * segments of 1, 2, 4 or 8 consecutive 128-B lines of it, the access shape
+ * of the search's ion-id stream. A measured denominator for the search's
+ * roofline beside the published HBM peak; no reference counterpart. */
+int gx_measure_read_bandwidth(gx_ctx *ctx, int segment_lines, double *gbps);
+
 /* Text of the last failure on this thread: for GX_ERR_DEVICE the call, its
  * source line and the HIP error string (the reference prints the CUDA error
  * and exits, cuda/driver.hpp:41-48); for a rejected search argument, which

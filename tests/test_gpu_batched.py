@@ -82,3 +82,17 @@ def test_autotune_choice_keeps_parity(open_index, searcher_open):
     assert st["config"] == r["config"]
     c, hc, _ = O.search(open_index, batch, O.default_params(dM=500.0), want_hist=True)
     compare(g, c, hg, hc, "autotuned")
+
+
+def test_measured_read_bandwidth(searcher_small):
+    """The bench's measured denominators: sequential and random-segment reads
+    report plausible rates (a few TB/s, below the 8 TB/s published peak);
+    other segment sizes are rejected."""
+    say("case: measured read bandwidth")
+    seq = searcher_small.read_bandwidth(0)
+    rnd = searcher_small.read_bandwidth(2)
+    say(f"  sequential {seq:.0f} GB/s, random 2 x 128 B segments {rnd:.0f} GB/s")
+    assert 500.0 < seq < 8000.0 and 200.0 < rnd < 8000.0
+    with pytest.raises(F.GxError) as e:
+        searcher_small.read_bandwidth(3)
+    assert e.value.code == F.GX_ERR_INVLD_PARAM
