@@ -393,10 +393,13 @@ __device__ __forceinline__ void merge_best(unsigned long long &k, uint32_t &e, u
 // 2^cell_shift-bin cell) kept in the tail of the count tile.
 template <int THREADS, int TILE_WORDS, int SL>
 __device__ void rescore_candidates(const ItemArgs &a, SpecLds<THREADS, TILE_WORDS, SL> &L, int P, uint32_t *g_cand,
-                                   unsigned long long &bkey, uint32_t &beq, uint32_t &bshp, uint32_t &bcps,
                                    uint32_t &n_err) {
     constexpr int CB = (TILE_WORDS - kPeakIdxWords) / 3;  // candidates per round (3 accumulators each)
     const int tid = threadIdx.x;
+    // this call's top hit and candidate count; merged into the wave's slot at
+    // the end, so that nothing of it stays live in registers between calls
+    unsigned long long bkey = 0;
+    uint32_t beq = 0, bshp = 0, bcps = 0;
     const int dF = a.dF;
     const uint32_t ncand = L.n_cand;
     uint16_t *first = reinterpret_cast<uint16_t *>(L.cnt + (TILE_WORDS - kPeakIdxWords));
@@ -474,6 +477,18 @@ __device__ void rescore_candidates(const ItemArgs &a, SpecLds<THREADS, TILE_WORD
     }
     for (uint32_t c = tid; c < (ncell + 1) / 2; c += THREADS) L.cnt[TILE_WORDS - kPeakIdxWords + c] = 0;
     if (tid == 0) L.n_cand = 0;
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        const unsigned long long k2 = __shfl_xor(bkey, o, 64);
+        const uint32_t e2 = __shfl_xor(beq, o, 64), s2 = __shfl_xor(bshp, o, 64);
+        merge_best(bkey, beq, bshp, k2, e2, s2);
+    }
+    bcps = wave_sum_u32(bcps);
+    if ((tid & 63) == 0) {
+        const int wave = tid >> 6;
+        merge_best(L.w_key[wave], L.w_eq[wave], L.w_shp[wave], bkey, beq, bshp);
+        L.w_cps[wave] += bcps;
+    }
     lds_barrier();
 }
 
@@ -522,8 +537,14 @@ __global__ void __launch_bounds__(THREADS, WAVES_PER_SIMD) search_spectra_kernel
             L.pk_i[k] = a.li[qoff + k];
         }
         const int nsl = P * nb5;
-        unsigned long long bkey = 0;
-        uint32_t beq = 0, bshp = 0, bcps = 0;
+        // the unit's top hit and candidate count, per wave (merged by every
+        // pass-2 call; ordered before the first one by the barriers between)
+        if (lane == 0) {
+            L.w_key[wave] = 0;
+            L.w_eq[wave] = 0;
+            L.w_shp[wave] = 0;
+            L.w_cps[wave] = 0;
+        }
         uint32_t n_scanned = 0, n_errors = 0;
         const uint64_t pos_s = a.pos_off[s] - a.pos_off[a.chunk_s0];
         // the spectrum's per-bucket plan, bucket b in lane b of every wave (one
@@ -579,7 +600,7 @@ __global__ void __launch_bounds__(THREADS, WAVES_PER_SIMD) search_spectra_kernel
                 // pass 2 reads candidates that pass 1 wrote to global scratch)
                 __syncthreads();
                 if (L.n_cand + nent > cand_cap)
-                    rescore_candidates<THREADS, TILE_WORDS, SL>(a, L, P, g_cand, bkey, beq, bshp, bcps, n_errors);
+                    rescore_candidates<THREADS, TILE_WORDS, SL>(a, L, P, g_cand, n_errors);
                 // ---- pass 1: shared-peak counts of every entry in the tile ----
                 for (int c0 = 0; c0 < nsl; c0 += SL) {
                     const int ns = min(SL, nsl - c0);
@@ -700,11 +721,13 @@ __global__ void __launch_bounds__(THREADS, WAVES_PER_SIMD) search_spectra_kernel
                                 // ids of the tile's entries are [tlo_id, tlo_id + span_id);
                                 // with the exact division (checked on the host for every
                                 // id < 2^31) that gives 0 <= el < nent
-                                if (raw[u] - tlo_id < span_id) {
+                                const bool in_tile = raw[u] - tlo_id < span_id;
+                                // the wave's count of consumed ids (uniform: kept in a scalar register)
+                                n_scanned += (uint32_t)__popcll(__ballot(in_tile));
+                                if (in_tile) {
                                     const uint32_t el = (__umulhi(raw[u], dm) >> dsh) - (uint32_t)tlo;
                                     const uint32_t sh = (el & (pack - 1)) << fsh;
                                     const uint32_t old = atomicAdd(&L.cnt[el >> lp], 1u << sh);
-                                    n_scanned++;
                                     // the increment that reaches min_shp makes the entry a
                                     // candidate, exactly once
                                     if (((old >> sh) & fmask) == min_cnt - 1u) {
@@ -732,23 +755,10 @@ __global__ void __launch_bounds__(THREADS, WAVES_PER_SIMD) search_spectra_kernel
             }
         }
         __syncthreads();  // candidates in global scratch, see above
-        if (L.n_cand) rescore_candidates<THREADS, TILE_WORDS, SL>(a, L, P, g_cand, bkey, beq, bshp, bcps, n_errors);
-        my_scanned += n_scanned;
+        if (L.n_cand) rescore_candidates<THREADS, TILE_WORDS, SL>(a, L, P, g_cand, n_errors);
+        my_scanned += n_scanned;  // already the wave's total
         my_errors += n_errors;
-        // ---- block reduction of the top hit; the spectrum's results ----
-#pragma unroll
-        for (int o = 32; o > 0; o >>= 1) {
-            const unsigned long long k2 = __shfl_xor(bkey, o, 64);
-            const uint32_t e2 = __shfl_xor(beq, o, 64), s2 = __shfl_xor(bshp, o, 64);
-            merge_best(bkey, beq, bshp, k2, e2, s2);
-        }
-        bcps = wave_sum_u32(bcps);
-        if (lane == 0) {
-            L.w_key[wave] = bkey;
-            L.w_eq[wave] = beq;
-            L.w_shp[wave] = bshp;
-            L.w_cps[wave] = bcps;
-        }
+        // ---- block reduction of the top hit; the unit's results ----
         lds_barrier();
         const bool whole = slot == kWholeUnit;
         uint32_t *hout = whole ? a.hist + (size_t)s * GX_HIST_BINS : a.part_hist + (size_t)slot * GX_HIST_BINS;
@@ -778,7 +788,6 @@ __global__ void __launch_bounds__(THREADS, WAVES_PER_SIMD) search_spectra_kernel
         }
     }
     if (tid == 0) stat_add(a.stats, ST_CAND, my_cand);
-    my_scanned = wave_sum_u64(my_scanned);
     my_errors = wave_sum_u64(my_errors);
     if (lane == 0) {
         // H (window-restricted ion ids) is counted where they are consumed
