@@ -244,7 +244,7 @@ def main() -> int:
     extra = {}
     if engine == F.GX_ENGINE_BATCHED and args.config < 0:
         # the project's autotuner on a sample of the batch (launch configuration x window split)
-        ns = min(batch.n_spectra, 10000)
+        ns = min(batch.n_spectra, 25000)
         sample = s.upload(batch.subset(np.arange(ns)))
         t = time.perf_counter()
         tune = s.autotune(sample, params)

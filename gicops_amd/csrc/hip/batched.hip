@@ -347,8 +347,6 @@ __global__ void __launch_bounds__(kSliceThreads) plan_slices_kernel(PlanArgs a) 
 template <int THREADS, int TILE_WORDS, int SL>
 struct SpecLds {
     alignas(16) uint32_t cnt[TILE_WORDS];  // count tile; pass-2 accumulators and the peak index after pass 1
-    int32_t pk_q[kPeakCap];
-    int32_t pk_i[kPeakCap];
     uint32_t s_a[SL];          // non-empty slices of the tile: first ion
     uint32_t s_flat[SL + 1];   // exclusive prefix of their lengths
     uint32_t wsum[THREADS / 64];
@@ -392,10 +390,21 @@ __device__ __forceinline__ void merge_best(unsigned long long &k, uint32_t &e, u
 // near a bin are found through a coarse index (first peak of each
 // 2^cell_shift-bin cell) kept in the tail of the count tile.
 template <int THREADS, int TILE_WORDS, int SL>
-__device__ void rescore_candidates(const ItemArgs &a, SpecLds<THREADS, TILE_WORDS, SL> &L, int P, uint32_t *g_cand,
-                                   uint32_t &n_err) {
-    constexpr int CB = (TILE_WORDS - kPeakIdxWords) / 3;  // candidates per round (3 accumulators each)
+__device__ void rescore_candidates(const ItemArgs &a, SpecLds<THREADS, TILE_WORDS, SL> &L, int P, uint32_t qoff,
+                                   uint32_t *g_cand, uint32_t &n_err) {
+    // the tile during pass 2: accumulators, the spectrum's peaks (m/z bins and
+    // intensities, kPeakCap each), the coarse peak index
+    constexpr int kPeakWords = TILE_WORDS - kPeakIdxWords - 2 * kPeakCap;
+    static_assert(kPeakWords > 0, "count tile too small for pass 2");
+    constexpr int CB = kPeakWords / 3;  // candidates per round (3 accumulators each)
     const int tid = threadIdx.x;
+    int32_t *pk_q = reinterpret_cast<int32_t *>(L.cnt + kPeakWords);
+    int32_t *pk_i = pk_q + kPeakCap;
+    for (int k = tid; k < P; k += THREADS) {
+        pk_q[k] = a.lq[qoff + k];
+        pk_i[k] = a.li[qoff + k];
+    }
+    lds_barrier();
     // this call's top hit and candidate count; merged into the wave's slot at
     // the end, so that nothing of it stays live in registers between calls
     unsigned long long bkey = 0;
@@ -405,7 +414,7 @@ __device__ void rescore_candidates(const ItemArgs &a, SpecLds<THREADS, TILE_WORD
     uint16_t *first = reinterpret_cast<uint16_t *>(L.cnt + (TILE_WORDS - kPeakIdxWords));
     const uint32_t cs = a.cell_shift;
     const uint32_t ncell = ((uint32_t)a.nbins >> cs) + 1;
-    for (uint32_t c = tid; c < ncell; c += THREADS) first[c] = (uint16_t)lower_bound_lds(L.pk_q, P, (int32_t)(c << cs));
+    for (uint32_t c = tid; c < ncell; c += THREADS) first[c] = (uint16_t)lower_bound_lds(pk_q, P, (int32_t)(c << cs));
     uint32_t *acc_n = L.cnt, *acc_ib = L.cnt + CB, *acc_iy = L.cnt + 2 * CB;
     const uint32_t smax = a.max_speclen;
     lds_barrier();
@@ -436,11 +445,11 @@ __device__ void rescore_candidates(const ItemArgs &a, SpecLds<THREADS, TILE_WORD
                 if (cl[u] == 0xffffffffu) continue;
                 const int32_t lo = x[u] - dF, hi = x[u] + dF;
                 int k = first[(uint32_t)max(lo, 0) >> cs];
-                while (k < P && L.pk_q[k] < lo) k++;
+                while (k < P && pk_q[k] < lo) k++;
                 uint32_t n = 0, sum = 0;
-                for (; k < P && L.pk_q[k] <= hi; k++) {
+                for (; k < P && pk_q[k] <= hi; k++) {
                     n++;
-                    sum += (uint32_t)L.pk_i[k];
+                    sum += (uint32_t)pk_i[k];
                 }
                 if (n) {
                     const bool isy = rr[u] >= (sl[u] >> 1);
@@ -476,6 +485,10 @@ __device__ void rescore_candidates(const ItemArgs &a, SpecLds<THREADS, TILE_WORD
         lds_barrier();
     }
     for (uint32_t c = tid; c < (ncell + 1) / 2; c += THREADS) L.cnt[TILE_WORDS - kPeakIdxWords + c] = 0;
+    for (int k = tid; k < P; k += THREADS) {
+        pk_q[k] = 0;
+        pk_i[k] = 0;
+    }
     if (tid == 0) L.n_cand = 0;
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) {
@@ -531,11 +544,7 @@ __global__ void __launch_bounds__(THREADS, WAVES_PER_SIMD) search_spectra_kernel
         // fetch the next ticket now; its latency overlaps this spectrum
         if (tid == 0) L.next = atomicAdd(a.work_counter, 1u);
         const int P = (int)a.n_legal[s];
-        const uint32_t qoff = a.qidx[s];
-        for (int k = tid; k < P; k += THREADS) {
-            L.pk_q[k] = a.lq[qoff + k];
-            L.pk_i[k] = a.li[qoff + k];
-        }
+        const uint32_t qoff = a.qidx[s];  // the legal peaks, read by pass 2
         const int nsl = P * nb5;
         // the unit's top hit and candidate count, per wave (merged by every
         // pass-2 call; ordered before the first one by the barriers between)
@@ -600,7 +609,7 @@ __global__ void __launch_bounds__(THREADS, WAVES_PER_SIMD) search_spectra_kernel
                 // pass 2 reads candidates that pass 1 wrote to global scratch)
                 __syncthreads();
                 if (L.n_cand + nent > cand_cap)
-                    rescore_candidates<THREADS, TILE_WORDS, SL>(a, L, P, g_cand, n_errors);
+                    rescore_candidates<THREADS, TILE_WORDS, SL>(a, L, P, qoff, g_cand, n_errors);
                 // ---- pass 1: shared-peak counts of every entry in the tile ----
                 for (int c0 = 0; c0 < nsl; c0 += SL) {
                     const int ns = min(SL, nsl - c0);
@@ -755,7 +764,7 @@ __global__ void __launch_bounds__(THREADS, WAVES_PER_SIMD) search_spectra_kernel
             }
         }
         __syncthreads();  // candidates in global scratch, see above
-        if (L.n_cand) rescore_candidates<THREADS, TILE_WORDS, SL>(a, L, P, g_cand, n_errors);
+        if (L.n_cand) rescore_candidates<THREADS, TILE_WORDS, SL>(a, L, P, qoff, g_cand, n_errors);
         my_scanned += n_scanned;  // already the wave's total
         my_errors += n_errors;
         // ---- block reduction of the top hit; the unit's results ----
@@ -895,12 +904,13 @@ hipError_t launch_spectra(const ItemArgs &a, int n_blocks, hipStream_t s) {
 
 }  // namespace
 
-// LDS per block: 4*TILE_WORDS + 16 KB peaks + 12*SL + 4 KB candidates + 3.2 KB histogram
+// LDS per block: 4*TILE_WORDS + 13*SL + 2.5 KB candidates + 3.2 KB histogram
+// (the spectrum's peaks live in the count tile during pass 2)
 BatchConfig batch_config(int id) {
     switch (id) {
         case 0: return {1024, 28672, 1024, 1};  // 112 KB tile, one block per CU
-        case 1: return {512, 12288, 512, 2};    // 48 KB tile, two blocks per CU
-        case 2: return {256, 4096, 256, 3};     // 16 KB tile, three blocks per CU
+        case 1: return {512, 16384, 512, 2};    // 64 KB tile, two blocks per CU
+        case 2: return {256, 8192, 256, 3};     // 32 KB tile, three blocks per CU
         default: return {768, 28672, 768, 1};   // 112 KB tile, one block of 12 waves per CU
     }
 }
@@ -934,8 +944,8 @@ hipError_t launch_plan_slices(const PlanArgs &a, hipStream_t s) {
 hipError_t launch_search_spectra(int config, const ItemArgs &a, int n_blocks, hipStream_t s) {
     switch (config) {
         case 0: return launch_spectra<1024, 28672, 1024, 4>(a, n_blocks, s);
-        case 1: return launch_spectra<512, 12288, 512, 4>(a, n_blocks, s);
-        case 2: return launch_spectra<256, 4096, 256, 3>(a, n_blocks, s);
+        case 1: return launch_spectra<512, 16384, 512, 4>(a, n_blocks, s);
+        case 2: return launch_spectra<256, 8192, 256, 3>(a, n_blocks, s);
         default: return launch_spectra<768, 28672, 768, 3>(a, n_blocks, s);
     }
 }

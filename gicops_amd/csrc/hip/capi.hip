@@ -456,7 +456,7 @@ int grow(T **p, size_t &cap, size_t n, uint64_t &gen) {
     return st;
 }
 
-constexpr uint64_t kPosChunkWords = 1ull << 31;  // 8 GB of slice positions per planning chunk
+constexpr uint64_t kPosChunkWords = 1ull << 32;  // 16 GB of slice positions per planning chunk
 
 struct Batched {
     int cfg_id;
@@ -1024,22 +1024,26 @@ extern "C" int gx_autotune(gx_ctx *c, const gx_dindex *idx, gx_dqueries *q, cons
         return GX_SUCCESS;
     };
     // launch configuration x window split at the current threshold, then the
-    // threshold for the best pair
+    // threshold for the two best pairs (two launch shapes can be within a few
+    // percent of each other at one threshold and apart at another)
     const int splits[] = {1, 2};
     const int w0 = bw;
+    std::vector<std::pair<double, std::pair<int, int>>> first;
     for (int cfg = 0; cfg < kNumBatchConfigs; cfg++)
         for (int sp : splits) {
             double t;
             const int r = trial(cfg, sp, w0, t);
             if (r) return r;
+            first.push_back({t, {cfg, sp}});
         }
-    const int cfg1 = bc, sp1 = bs;
-    for (int wb : {32, 64, 128, 256}) {
-        if (wb == w0) continue;
-        double t;
-        const int r = trial(cfg1, sp1, wb, t);
-        if (r) return r;
-    }
+    std::sort(first.begin(), first.end());
+    for (size_t k = 0; k < std::min<size_t>(2, first.size()); k++)
+        for (int wb : {32, 64, 128, 256}) {
+            if (wb == w0) continue;
+            double t;
+            const int r = trial(first[k].second.first, first[k].second.second, wb, t);
+            if (r) return r;
+        }
     c->tuned_config = bc;
     c->tuned_split = bs;
     c->tuned_whole = bw;
