@@ -908,7 +908,7 @@ hipError_t launch_spectra(const ItemArgs &a, int n_blocks, hipStream_t s) {
 // (the spectrum's peaks live in the count tile during pass 2)
 BatchConfig batch_config(int id) {
     switch (id) {
-        case 0: return {1024, 28672, 1024, 1};  // 112 KB tile, one block per CU
+        case 0: return {1024, 32768, 1024, 1};  // 128 KB tile, one block per CU
         case 1: return {512, 16384, 512, 2};    // 64 KB tile, two blocks per CU
         case 2: return {256, 8192, 256, 3};     // 32 KB tile, three blocks per CU
         default: return {768, 28672, 768, 1};   // 112 KB tile, one block of 12 waves per CU
@@ -943,7 +943,7 @@ hipError_t launch_plan_slices(const PlanArgs &a, hipStream_t s) {
 
 hipError_t launch_search_spectra(int config, const ItemArgs &a, int n_blocks, hipStream_t s) {
     switch (config) {
-        case 0: return launch_spectra<1024, 28672, 1024, 4>(a, n_blocks, s);
+        case 0: return launch_spectra<1024, 32768, 1024, 4>(a, n_blocks, s);
         case 1: return launch_spectra<512, 16384, 512, 4>(a, n_blocks, s);
         case 2: return launch_spectra<256, 8192, 256, 3>(a, n_blocks, s);
         default: return launch_spectra<768, 28672, 768, 3>(a, n_blocks, s);
