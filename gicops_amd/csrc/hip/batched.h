@@ -41,6 +41,7 @@ struct PlanArgs {
     uint32_t *pair_tiles;    // tiles of the pair (0 = no window)
     uint32_t *pair_pack;     // entries per count word (4 or 2)
     uint32_t *pair_pos_off;  // spectrum-local slice-position offset (words)
+    uint32_t *pair_nsl;      // non-empty slices of the pair (written by plan_slices)
     uint32_t *n_items;       // per spectrum, scanned in place to offsets by scan_kernel
     uint64_t *n_pos;         // per spectrum, scanned in place to offsets
     // scanned offsets (n_spectra + 1 entries)
@@ -71,6 +72,7 @@ struct ItemArgs {
     const uint32_t *pair_tiles;
     const uint32_t *pair_pack;
     const uint32_t *pair_pos_off;
+    const uint32_t *pair_nsl;
     const uint64_t *pos_off;
     const uint32_t *pos;
     uint32_t chunk_s0, chunk_s1;  // spectra of the chunk

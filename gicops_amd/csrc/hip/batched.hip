@@ -287,7 +287,10 @@ __global__ void __launch_bounds__(kSliceThreads) plan_slices_kernel(PlanArgs a) 
     const uint32_t b = blockIdx.x / nspec;
     const uint32_t pi = s * nb + b;
     const uint32_t tiles = a.pair_tiles[pi];
-    if (tiles == 0) return;
+    if (tiles == 0) {
+        if (tid == 0) a.pair_nsl[pi] = 0;
+        return;
+    }
     if (tid < 2) s_st[tid] = 0;
     __syncthreads();
     const int dF = a.dF, nb5 = 2 * dF + 1;
@@ -302,45 +305,73 @@ __global__ void __launch_bounds__(kSliceThreads) plan_slices_kernel(PlanArgs a) 
     const int32_t *lq = a.lq + a.qidx[s];
     unsigned long long bins = 0;
     // tile-major rows: row t holds boundary t of every slice, so that the
-    // search kernel's item for tile t reads rows t and t+1 coalesced
+    // search kernel's item for tile t reads rows t and t+1 coalesced. Only
+    // non-empty slices get a column (in any order: the search does not depend
+    // on it); their number goes to pair_nsl.
     uint32_t *out = a.pos + base;
-    for (int j = tid; j < nsl; j += kSliceThreads) {
-        const int k = j / nb5;
-        const uint32_t bin = (uint32_t)(lq[k] - dF + (j - k * nb5));
-        const uint32_t st = gptr(bk.bA)[bin], en = gptr(bk.bA)[bin + 1];
-        // bins of at most whole_bin ions are streamed whole by the search
-        // kernel: a flagged start in rows 0..tiles-1 and the end in the last
-        // row (every tile filters by id); larger ones get boundary t = first
-        // ion of the slice with id >= (lo + t*T)*speclen, the last one being
-        // the window end
-        if (en - st <= a.whole_bin) {
-            for (uint32_t t = 0; t < tiles; t++) out[(uint64_t)t * nsl + j] = st | kWholeFlag;
-            out[(uint64_t)tiles * nsl + j] = en;
+    const unsigned long long below = lane ? ((1ull << (uint32_t)lane) - 1ull) : 0ull;
+    for (int j0 = 0; j0 < nsl; j0 += kSliceThreads) {
+        const int j = j0 + tid;
+        const bool act = j < nsl;
+        uint32_t st = 0, en = 0, c0 = 0, cz = 0;
+        bool whole = false;
+        if (act) {
+            const int k = j / nb5;
+            const uint32_t bin = (uint32_t)(lq[k] - dF + (j - k * nb5));
+            st = gptr(bk.bA)[bin];
+            en = gptr(bk.bA)[bin + 1];
             bins++;
-            continue;
-        }
-        // boundaries two at a time, each pair searched from the previous one
-        uint32_t cur = st;
-        for (uint32_t t = 0; t <= tiles; t += 2) {
-            const uint32_t id0 = (t == tiles) ? end_id : (lo + t * T) * speclen;
-            if (t + 1 <= tiles) {
-                const uint32_t id1 = (t + 1 == tiles) ? end_id : (lo + (t + 1) * T) * speclen;
-                uint32_t c0 = cur, c1 = cur;
-                if (en > cur) lower_bound_sampled2(bk, cur, en, id0, id1, c0, c1);
-                out[(uint64_t)t * nsl + j] = c0;
-                out[(uint64_t)(t + 1) * nsl + j] = c1;
-                cur = c1;
-            } else {
-                if (en > cur) cur = lower_bound_sampled(bk, cur, en, id0);
-                out[(uint64_t)t * nsl + j] = cur;
+            // bins of at most whole_bin ions are streamed whole by the search
+            // kernel (every tile filters by id); larger ones get boundary
+            // t = first ion of the slice with id >= (lo + t*T)*speclen, the
+            // last one being the window end. First the window's two ends.
+            whole = en - st <= a.whole_bin;
+            if (!whole) {
+                const uint32_t id0 = lo * speclen;
+                c0 = st;
+                cz = st;
+                lower_bound_sampled2(bk, st, en, id0, end_id, c0, cz);
             }
         }
-        bins++;
+        const bool keep = act && (whole ? en > st : cz > c0);
+        // the column of each kept slice: a wave-aggregated LDS counter
+        const unsigned long long km = __ballot(keep);
+        uint32_t wb0 = 0;
+        if (lane == 0 && km) wb0 = (uint32_t)atomicAdd(&s_st[0], (unsigned long long)__popcll(km));
+        wb0 = (uint32_t)__shfl(wb0, 0, 64);
+        if (!keep) continue;
+        const uint32_t jc = wb0 + (uint32_t)__popcll(km & below);
+        if (whole) {
+            for (uint32_t t = 0; t < tiles; t++) out[(uint64_t)t * nsl + jc] = st | kWholeFlag;
+            out[(uint64_t)tiles * nsl + jc] = en;
+            continue;
+        }
+        out[jc] = c0;
+        out[(uint64_t)tiles * nsl + jc] = cz;
+        // interior boundaries two at a time, each pair searched from the previous one
+        uint32_t cur = c0;
+        for (uint32_t t = 1; t < tiles; t += 2) {
+            const uint32_t id0 = (lo + t * T) * speclen;
+            if (t + 1 < tiles) {
+                const uint32_t id1 = (lo + (t + 1) * T) * speclen;
+                uint32_t b0 = cur, b1 = cur;
+                if (cz > cur) lower_bound_sampled2(bk, cur, cz, id0, id1, b0, b1);
+                out[(uint64_t)t * nsl + jc] = b0;
+                out[(uint64_t)(t + 1) * nsl + jc] = b1;
+                cur = b1;
+            } else {
+                if (cz > cur) cur = lower_bound_sampled(bk, cur, cz, id0);
+                out[(uint64_t)t * nsl + jc] = cur;
+            }
+        }
     }
     bins = wave_sum_u64(bins);
     if (lane == 0) atomicAdd(&s_st[1], bins);
     __syncthreads();
-    if (tid == 0) stat_add(a.stats, ST_BINS, s_st[1]);
+    if (tid == 0) {
+        stat_add(a.stats, ST_BINS, s_st[1]);
+        a.pair_nsl[pi] = (uint32_t)s_st[0];
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -558,24 +589,28 @@ __global__ void __launch_bounds__(THREADS, WAVES_PER_SIMD) search_spectra_kernel
         const uint64_t pos_s = a.pos_off[s] - a.pos_off[a.chunk_s0];
         // the spectrum's per-bucket plan, bucket b in lane b of every wave (one
         // round trip per spectrum instead of a chain per item)
-        uint32_t w_tiles = 0, w_pack = 0, w_poff = 0;
+        uint32_t w_tiles = 0, w_pack = 0, w_poff = 0, w_nsl = 0;
         int32_t w_lo = 0, w_hi = -1;
         if ((uint32_t)lane < nbk) {
             const uint32_t pi = s * nbk + (uint32_t)lane;
             w_tiles = a.pair_tiles[pi];
             w_pack = a.pair_pack[pi];
             w_poff = a.pair_pos_off[pi];
+            w_nsl = a.pair_nsl[pi];
             w_lo = a.pair_lo[pi];
             w_hi = a.pair_hi[pi];
         }
         // the slice words of the next (bucket, tile, slice chunk) step are
-        // loaded one step ahead, so that their latency overlaps this step
+        // loaded one step ahead, so that their latency overlaps this step. A
+        // pair's rows have nsl columns, of which the first w_nsl (its non-empty
+        // slices) are filled.
         uint32_t nx_pt = 0, nx_pt1 = 0, nx_pe = 0;
         auto fetch = [&](uint32_t fb, uint32_t ft, int fc0) {
             const uint32_t ftiles = (uint32_t)__builtin_amdgcn_readlane((int)w_tiles, (int)fb);
             const uint32_t fpoff = (uint32_t)__builtin_amdgcn_readlane((int)w_poff, (int)fb);
+            const int fnsl = __builtin_amdgcn_readlane((int)w_nsl, (int)fb);
             const uint32_t *fpos = a.pos + pos_s + fpoff;
-            if (tid < min(SL, nsl - fc0)) {
+            if (tid < min(SL, fnsl - fc0)) {
                 const uint32_t j = (uint32_t)(fc0 + tid);
                 nx_pt = fpos[(uint64_t)ft * nsl + j];
                 nx_pt1 = fpos[(uint64_t)(ft + 1) * nsl + j];
@@ -586,6 +621,12 @@ __global__ void __launch_bounds__(THREADS, WAVES_PER_SIMD) search_spectra_kernel
         if (todo0) fetch((uint32_t)__builtin_ctzll(todo0), 0, 0);
         for (unsigned long long todo = todo0; todo; todo &= todo - 1) {
             const uint32_t b = (uint32_t)__builtin_ctzll(todo);
+            const int nslb = __builtin_amdgcn_readlane((int)w_nsl, (int)b);  // non-empty slices
+            if (nslb == 0) {  // nothing of the window in this spectrum's bins
+                const unsigned long long rest = todo & (todo - 1);
+                if (rest) fetch((uint32_t)__builtin_ctzll(rest), 0, 0);
+                continue;
+            }
             const uint32_t tiles = (uint32_t)__builtin_amdgcn_readlane((int)w_tiles, (int)b);
             const int32_t lo = __builtin_amdgcn_readlane(w_lo, (int)b), hi = __builtin_amdgcn_readlane(w_hi, (int)b);
             const uint32_t pack = (uint32_t)__builtin_amdgcn_readlane((int)w_pack, (int)b);
@@ -611,15 +652,15 @@ __global__ void __launch_bounds__(THREADS, WAVES_PER_SIMD) search_spectra_kernel
                 if (L.n_cand + nent > cand_cap)
                     rescore_candidates<THREADS, TILE_WORDS, SL>(a, L, P, qoff, g_cand, n_errors);
                 // ---- pass 1: shared-peak counts of every entry in the tile ----
-                for (int c0 = 0; c0 < nsl; c0 += SL) {
-                    const int ns = min(SL, nsl - c0);
+                for (int c0 = 0; c0 < nslb; c0 += SL) {
+                    const int ns = min(SL, nslb - c0);
                     // slice ranges of this tile; non-empty ones compacted in order
                     static_assert(SL == THREADS, "one slice per thread");
                     uint32_t at = 0, len = 0;
                     // this step's words (rows t, t+1 and the last of its slices),
                     // then the next step's load
                     const uint32_t pt = nx_pt, pt1 = nx_pt1, pe = nx_pe;
-                    if (c0 + SL < nsl) {
+                    if (c0 + SL < nslb) {
                         fetch(b, t, c0 + SL);
                     } else if (t + 1 < tiles) {
                         fetch(b, t + 1, 0);

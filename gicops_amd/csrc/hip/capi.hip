@@ -321,7 +321,7 @@ extern "C" void gx_queries_free(gx_dqueries *d) {
     if (d->graph) (void)hipGraphDestroy(d->graph);
     void *ps[] = {d->idx,       d->moz,        d->inten,         d->prec,         d->dm,       d->psm,
                   d->hist,      d->lq,         d->li,            d->n_legal,      d->pair_lo,  d->pair_hi,
-                  d->pair_tiles, d->pair_pack, d->pair_pos_off, d->n_items,     d->item_off, d->n_pos,
+                  d->pair_tiles, d->pair_pack, d->pair_pos_off, d->pair_nsl, d->n_items,     d->item_off, d->n_pos,
                   d->pos_off,   d->cpsms,      d->counters,      d->best_key,     d->best_eq,  d->best_shp,
                   d->pos,       d->unit_s,     d->unit_br,       d->fold_s,       d->fold_p0,  d->fold_n,
                   d->part_key,  d->part_eq,    d->part_shp,      d->part_cps,     d->part_hist, d->porder,           d->l_feat,       d->l_h1,     d->l_h2,
@@ -426,7 +426,7 @@ int ensure_batched(gx_dqueries *q, const gx_dindex *idx) {
     q->nb = idx->n_buckets;
     if (dalloc(&q->lq, np) || dalloc(&q->li, np) || dalloc(&q->n_legal, ns) || dalloc(&q->pair_lo, npairs) ||
         dalloc(&q->pair_hi, npairs) || dalloc(&q->pair_tiles, npairs) || dalloc(&q->pair_pack, npairs) ||
-        dalloc(&q->pair_pos_off, npairs) || dalloc(&q->n_items, ns) || dalloc(&q->best_key, ns) || dalloc(&q->porder, ns) ||
+        dalloc(&q->pair_pos_off, npairs) || dalloc(&q->pair_nsl, npairs) || dalloc(&q->n_items, ns) || dalloc(&q->best_key, ns) || dalloc(&q->porder, ns) ||
         dalloc(&q->best_eq, ns) || dalloc(&q->best_shp, ns) ||
         dalloc(&q->item_off, ns + 1) || dalloc(&q->n_pos, ns) || dalloc(&q->pos_off, ns + 1) ||
         dalloc(&q->cpsms, ns) || (!q->hist && dalloc(&q->hist, ns * GX_HIST_BINS)))
@@ -498,6 +498,7 @@ Batched setup_batched(gx_ctx *c, const gx_dindex *idx, gx_dqueries *q, const gx_
     pa.pair_tiles = q->pair_tiles;
     pa.pair_pack = q->pair_pack;
     pa.pair_pos_off = q->pair_pos_off;
+    pa.pair_nsl = q->pair_nsl;
     pa.n_items = q->n_items;
     pa.n_pos = q->n_pos;
     pa.item_off = q->item_off;
@@ -528,6 +529,7 @@ void fill_item_args(Batched &B, gx_ctx *c, const gx_dindex *idx, gx_dqueries *q,
     ia.pair_tiles = q->pair_tiles;
     ia.pair_pack = q->pair_pack;
     ia.pair_pos_off = q->pair_pos_off;
+    ia.pair_nsl = q->pair_nsl;
     ia.pos_off = q->pos_off;
     ia.pos = q->pos;
     ia.unit_s = q->unit_s;

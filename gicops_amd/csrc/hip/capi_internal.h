@@ -78,7 +78,7 @@ struct gx_dqueries {
     int32_t *lq = nullptr, *li = nullptr;
     uint32_t *n_legal = nullptr;
     int32_t *pair_lo = nullptr, *pair_hi = nullptr;
-    uint32_t *pair_tiles = nullptr, *pair_pack = nullptr, *pair_pos_off = nullptr;
+    uint32_t *pair_tiles = nullptr, *pair_pack = nullptr, *pair_pos_off = nullptr, *pair_nsl = nullptr;
     uint32_t *n_items = nullptr, *item_off = nullptr;
     uint64_t *n_pos = nullptr, *pos_off = nullptr;
     uint32_t *cpsms = nullptr;
