@@ -300,6 +300,7 @@ __global__ void __launch_bounds__(kSliceThreads) plan_slices_kernel(PlanArgs a) 
     const uint32_t speclen = bk.speclen;
     const uint32_t lo = (uint32_t)a.pair_lo[pi];
     const uint32_t end_id = (uint32_t)(a.pair_hi[pi] + 1) * speclen;
+    const bool from_start = lo == 0, to_end = (uint32_t)(a.pair_hi[pi] + 1) == bk.n_entries;
     const uint32_t T = a.pair_pack[pi] * (uint32_t)a.tile_words;  // psids per tile
     const uint64_t base = (a.pos_off[s] - a.pos_off[a.chunk_s0]) + a.pair_pos_off[pi];
     const int32_t *lq = a.lq + a.qidx[s];
@@ -327,10 +328,17 @@ __global__ void __launch_bounds__(kSliceThreads) plan_slices_kernel(PlanArgs a) 
             // last one being the window end. First the window's two ends.
             whole = en - st <= a.whole_bin;
             if (!whole) {
+                // a window from the bucket's first entry starts at the bin's
+                // start, one to its last entry ends at the bin's end
                 const uint32_t id0 = lo * speclen;
                 c0 = st;
-                cz = st;
-                lower_bound_sampled2(bk, st, en, id0, end_id, c0, cz);
+                cz = en;
+                if (!from_start && !to_end)
+                    lower_bound_sampled2(bk, st, en, id0, end_id, c0, cz);
+                else if (!from_start)
+                    c0 = lower_bound_sampled(bk, st, en, id0);
+                else if (!to_end)
+                    cz = lower_bound_sampled(bk, st, en, end_id);
             }
         }
         const bool keep = act && (whole ? en > st : cz > c0);
