@@ -275,6 +275,47 @@ __device__ __forceinline__ void lower_bound_sampled2(const DevBucket &bk, uint32
     r1 = lo1;
 }
 
+// brackets of the first positions in [st, en) with iA[p] >= x0 and >= x1 from
+// the bin's samples alone: p0 in [lo0, hi0], p1 in [lo1, hi1], each bracket
+// within one 32-id block (one 128 B line of iA)
+__device__ __forceinline__ void bracket_sampled2(const DevBucket &bk, uint32_t st, uint32_t en, uint32_t x0, uint32_t x1,
+                                                 uint32_t &lo0, uint32_t &hi0, uint32_t &lo1, uint32_t &hi1) {
+    lo0 = st;
+    hi0 = en;
+    lo1 = st;
+    hi1 = en;
+    const uint32_t k0 = (st + 31u) >> 5;
+    if (en > 0 && k0 <= ((en - 1u) >> 5)) {
+        const uint32_t k1 = (en - 1u) >> 5;
+        const auto is = gptr(bk.iS);
+        uint32_t b0 = k0, b1 = k0, n0 = k1 - k0 + 1u, n1 = n0;
+        while (n0 > 0 || n1 > 0) {
+            const uint32_t h0 = n0 >> 1, h1 = n1 >> 1;
+            const uint32_t v0 = n0 > 0 ? is[b0 + h0] : 0u, v1 = n1 > 0 ? is[b1 + h1] : 0u;
+            if (n0 > 0) {
+                if (v0 < x0) {
+                    b0 += h0 + 1u;
+                    n0 -= h0 + 1u;
+                } else {
+                    n0 = h0;
+                }
+            }
+            if (n1 > 0) {
+                if (v1 < x1) {
+                    b1 += h1 + 1u;
+                    n1 -= h1 + 1u;
+                } else {
+                    n1 = h1;
+                }
+            }
+        }
+        if (b0 <= k1) hi0 = min(hi0, b0 << 5);
+        if (b0 > k0) lo0 = max(lo0, ((b0 - 1u) << 5) + 1u);
+        if (b1 <= k1) hi1 = min(hi1, b1 << 5);
+        if (b1 > k0) lo1 = max(lo1, ((b1 - 1u) << 5) + 1u);
+    }
+}
+
 // one block per (bucket, spectrum) of the chunk, bucket-major and by precursor
 // mass within a bucket: the blocks in flight read one bucket's bA and iA with
 // similar windows, which stay in L2
@@ -328,17 +369,16 @@ __global__ void __launch_bounds__(kSliceThreads) plan_slices_kernel(PlanArgs a) 
             // last one being the window end. First the window's two ends.
             whole = en - st <= a.whole_bin;
             if (!whole) {
-                // a window from the bucket's first entry starts at the bin's
-                // start, one to its last entry ends at the bin's end
-                const uint32_t id0 = lo * speclen;
-                c0 = st;
-                cz = en;
-                if (!from_start && !to_end)
-                    lower_bound_sampled2(bk, st, en, id0, end_id, c0, cz);
-                else if (!from_start)
-                    c0 = lower_bound_sampled(bk, st, en, id0);
-                else if (!to_end)
-                    cz = lower_bound_sampled(bk, st, en, end_id);
+                // the window's ends are only bracketed, from the samples: the
+                // slice starts at most 31 ids early and ends at most 31 ids late,
+                // within the 128 B lines its exact ends touch, and the search
+                // kernel drops ids outside the tile. A window from the bucket's
+                // first entry starts at the bin's start, one to its last entry
+                // ends at the bin's end.
+                uint32_t l0, h0, l1, h1;
+                bracket_sampled2(bk, st, en, lo * speclen, end_id, l0, h0, l1, h1);
+                c0 = from_start ? st : l0;
+                cz = to_end ? en : h1;
             }
         }
         const bool keep = act && (whole ? en > st : cz > c0);
