@@ -364,9 +364,9 @@ __global__ void __launch_bounds__(kSliceThreads) plan_slices_kernel(PlanArgs a) 
             en = gptr(bk.bA)[bin + 1];
             bins++;
             // bins of at most whole_bin ions are streamed whole by the search
-            // kernel (every tile filters by id); larger ones get boundary
-            // t = first ion of the slice with id >= (lo + t*T)*speclen, the
-            // last one being the window end. First the window's two ends.
+            // kernel (every tile filters by id). Larger ones get rows 0 and
+            // `tiles` from the window's ends (below) and, in between, row t =
+            // the first ion of the slice with id >= (lo + t*T)*speclen.
             whole = en - st <= a.whole_bin;
             if (!whole) {
                 // the window's ends are only bracketed, from the samples: the
