@@ -100,6 +100,34 @@ def test_uneven_batch_parity(open_index, searcher_open, eng):
     compare(g, c, hg, hc, "uneven")
 
 
+_WINDOW_CASES = {}
+
+
+@pytest.mark.parametrize("config", [0, 1])
+@pytest.mark.parametrize("dM, split, whole_bin", [(5000.0, 1, 32), (5000.0, 4, 1), (0.05, 1, 32), (500.0, 4, 1),
+                                                  (500.0, 2, 4096)])
+def test_window_extremes_parity(open_index, searcher_open, config, dM, split, whole_bin):
+    """Windows that span whole buckets (the plan needs no boundary search at
+    either end), windows of a few entries, windows cut into up to 4 tiles per
+    count tile (interior tile boundaries), every bin searched (whole-bin
+    threshold 1: every slice's ends bracketed from the 32x samples) and every
+    bin streamed whole (4096)."""
+    say(f"case: window extremes, dM {dM}, split {split}, whole bins <= {whole_bin}, config {config}")
+    cfg = slm.SpecConfig(n_spectra=120, seed=57, peak_mode=F.GX_PEAKS_TOTAL_UNIFORM, peaks_min=40,
+                         peaks_max=160, charge_min=2, charge_max=3, dm_choices=(dM,))
+    batch = slm.generate_spectra(open_index, cfg)
+    if dM not in _WINDOW_CASES:
+        _WINDOW_CASES[dM] = O.search(open_index, batch, O.default_params(dM=dM), want_hist=True,
+                                     want_counters=True)
+    c, hc, cnt = _WINDOW_CASES[dM]
+    p = SearchParams(dM=dM, engine=F.GX_ENGINE_BATCHED, config=config, split=split, whole_bin=whole_bin)
+    g, hg, st = searcher_open.search(batch, p, want_hist=True)
+    check_stats(st, cnt, batch.n_spectra)
+    compare(g, c, hg, hc, f"window extremes dM {dM}")
+    if dM >= 5000.0:  # the case's premise: every window is its whole bucket
+        assert st["window"] == batch.n_spectra * sum(b.n_entries for b in open_index.buckets)
+
+
 Z3_MODS = slm.DBConfig(n_peptides=20_000, seed=31, maxz=3,
                        mods=[("M", 15.9949, 2), ("STY", 79.96633, 1), ("^", 42.0106, 1)], max_mods_per_pep=3)
 
