@@ -471,13 +471,14 @@ __device__ __forceinline__ void merge_best(unsigned long long &k, uint32_t &e, u
 template <int THREADS, int TILE_WORDS, int SL>
 __device__ void rescore_candidates(const ItemArgs &a, SpecLds<THREADS, TILE_WORDS, SL> &L, int P, uint32_t qoff,
                                    uint32_t *g_cand, uint32_t &n_err) {
-    // the tile during pass 2: accumulators, the spectrum's peaks (m/z bins and
-    // intensities, kPeakCap each), the coarse peak index
-    constexpr int kPeakWords = TILE_WORDS - kPeakIdxWords - 2 * kPeakCap;
-    static_assert(kPeakWords > 0, "count tile too small for pass 2");
-    constexpr int CB = kPeakWords / 3;  // candidates per round (3 accumulators each)
+    // the tile during pass 2 (it is clear between items): accumulators, the
+    // spectrum's peaks (m/z bins and intensities, kPeakCap each, copied here
+    // per call), the coarse peak index
+    constexpr int kAccWords = TILE_WORDS - kPeakIdxWords - 2 * kPeakCap;
+    static_assert(kAccWords > 0, "count tile too small for pass 2");
+    constexpr int CB = kAccWords / 3;  // candidates per round (3 accumulators each)
     const int tid = threadIdx.x;
-    int32_t *pk_q = reinterpret_cast<int32_t *>(L.cnt + kPeakWords);
+    int32_t *pk_q = reinterpret_cast<int32_t *>(L.cnt + kAccWords);
     int32_t *pk_i = pk_q + kPeakCap;
     for (int k = tid; k < P; k += THREADS) {
         pk_q[k] = a.lq[qoff + k];
