@@ -9,8 +9,10 @@
 //                  LDS counters) and the number of psid tiles per window;
 //   scan           item and slice-position offsets per spectrum;
 //   plan_slices    one block per (bucket, spectrum): every (peak, bin) slice
-//                  of iA is cut to the window with two lower_bounds (small
-//                  bins are flagged to be streamed whole);
+//                  of iA is cut to the window (its ends bracketed from the
+//                  32x ion-id samples, interior tile boundaries searched
+//                  exactly; small bins are flagged to be streamed whole), and
+//                  only slices with ions left get slice positions;
 //   search_spectra persistent blocks pull spectra from a counter; each
 //                  (bucket, tile) item of the spectrum streams its slices'
 //                  ion ids in 64-id pieces (several loads in flight per wave)
