@@ -1040,7 +1040,7 @@ extern "C" int gx_autotune(gx_ctx *c, const gx_dindex *idx, gx_dqueries *q, cons
         }
     std::sort(first.begin(), first.end());
     for (size_t k = 0; k < std::min<size_t>(2, first.size()); k++)
-        for (int wb : {32, 64, 128, 256}) {
+        for (int wb : {8, 16, 32, 64, 128}) {
             if (wb == w0) continue;
             double t;
             const int r = trial(first[k].second.first, first[k].second.second, wb, t);

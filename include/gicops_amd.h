@@ -263,7 +263,7 @@ typedef struct gx_tune_result {
 
 /* Time every batched launch configuration x window split on a sample batch
  * (device-resident, as uploaded by gx_queries_upload), then the whole-bin
- * threshold for the two fastest pairs (14 trials), and make the winners the context's
+ * threshold (8, 16, 32, 64, 128 ions) for the two fastest pairs (16 trials), and make the winners the context's
  * defaults (params.config = -1, split = 0, whole_bin = 0). No reference
  * counterpart: the reference's launch shape is fixed (kernel.cu:588-598). */
 int gx_autotune(gx_ctx *ctx, const gx_dindex *idx, gx_dqueries *sample, const gx_search_params *p,

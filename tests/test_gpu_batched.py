@@ -75,9 +75,9 @@ def test_autotune_choice_keeps_parity(open_index, searcher_open):
     dq = searcher_open.upload(sample)
     r = searcher_open.autotune(dq, SearchParams(dM=500.0))
     dq.free()
-    assert r["config"] in (0, 1, 2, 3) and r["split"] in (1, 2) and r["whole_bin"] in (32, 64, 128, 256)
-    # 4 configurations x 2 splits, then 3 more thresholds for each of the two best pairs
-    assert len(r["tried"]) == 14 and all(ms > 0 for _, _, _, ms in r["tried"])
+    assert r["config"] in (0, 1, 2, 3) and r["split"] in (1, 2) and r["whole_bin"] in (8, 16, 32, 64, 128)
+    # 4 configurations x 2 splits, then 4 more thresholds for each of the two best pairs
+    assert len(r["tried"]) == 16 and all(ms > 0 for _, _, _, ms in r["tried"])
     first = sorted(r["tried"][:8], key=lambda x: x[3])[:2]
     swept = {(c, sp) for c, sp, _, _ in r["tried"][8:]}
     assert swept == {(c, sp) for c, sp, _, _ in first}
