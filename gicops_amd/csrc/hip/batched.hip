@@ -442,6 +442,11 @@ struct SpecLds {
     uint32_t w_shp[THREADS / 64];
     uint32_t w_cps[THREADS / 64];
     uint32_t b_speclen[kMaxBuckets];
+    // the candidates gathered since the last pass 2 come in runs, one per
+    // bucket (items go bucket by bucket): run k starts at seg_start[k]
+    uint32_t seg_start[kMaxBuckets];
+    uint8_t seg_b[kMaxBuckets];
+    uint32_t n_seg;
     uint32_t next;             // prefetched next ticket
     uint32_t n_cand;
 };
@@ -466,10 +471,11 @@ __device__ __forceinline__ void merge_best(unsigned long long &k, uint32_t &e, u
 
 // Pass 2 over the spectrum's candidates gathered so far: exact bc/yc/ibc/iyc
 // of each candidate from its own fragment bins (ion_bin) against the sorted
-// peaks, flattened over (candidate, fragment) with a max_speclen stride and
-// summed with LDS atomics; then hyperscore, histogram and top hit. Peaks
-// near a bin are found through a coarse index (first peak of each
-// 2^cell_shift-bin cell) kept in the tail of the count tile.
+// peaks, flattened over (candidate, fragment) run by run (a run's candidates
+// share a bucket, so its stride is that bucket's speclen) and summed with LDS
+// atomics; then hyperscore, histogram and top hit. Peaks near a bin are found
+// through a coarse index (first peak of each 2^cell_shift-bin cell) kept in
+// the tail of the count tile.
 template <int THREADS, int TILE_WORDS, int SL>
 __device__ void rescore_candidates(const ItemArgs &a, SpecLds<THREADS, TILE_WORDS, SL> &L, int P, uint32_t qoff,
                                    uint32_t *g_cand, uint32_t &n_err) {
@@ -498,45 +504,57 @@ __device__ void rescore_candidates(const ItemArgs &a, SpecLds<THREADS, TILE_WORD
     const uint32_t ncell = ((uint32_t)a.nbins >> cs) + 1;
     for (uint32_t c = tid; c < ncell; c += THREADS) first[c] = (uint16_t)lower_bound_lds(pk_q, P, (int32_t)(c << cs));
     uint32_t *acc_n = L.cnt, *acc_ib = L.cnt + CB, *acc_iy = L.cnt + 2 * CB;
-    const uint32_t smax = a.max_speclen;
+    const uint32_t nseg = (uint32_t)__builtin_amdgcn_readfirstlane((int)L.n_seg);
     lds_barrier();
     for (uint32_t c0 = 0; c0 < ncand; c0 += CB) {
         const uint32_t nc = min((uint32_t)CB, ncand - c0);
-        const uint32_t tasks = nc * smax;
-        constexpr int U = 4;
-        for (uint32_t tb = tid; tb < tasks; tb += U * THREADS) {
-            int32_t x[U];
-            uint32_t cl[U], rr[U], sl[U];
+        for (uint32_t k = 0; k < nseg; k++) {
+            // the run's candidates in this round, (candidate, fragment) tasks
+            // with the run's bucket's speclen as stride
+            const uint32_t sb = (uint32_t)__builtin_amdgcn_readfirstlane((int)L.seg_start[k]);
+            const uint32_t se =
+                k + 1 < nseg ? (uint32_t)__builtin_amdgcn_readfirstlane((int)L.seg_start[k + 1]) : ncand;
+            const uint32_t s0 = max(sb, c0), s1 = min(se, c0 + nc);
+            if (s1 <= s0) continue;
+            const uint32_t b = (uint32_t)__builtin_amdgcn_readfirstlane((int)L.seg_b[k]);
+            const DevBucket &bk = a.buckets[b];
+            const uint32_t spl = bk.speclen, dm = bk.div_m, dsh = bk.div_s;
+            const auto ib = gptr(bk.ion_bin);
+            const uint32_t nrun = s1 - s0, tasks = nrun * spl;  // < 2^31: the division below is exact
+            constexpr int U = 4;
+            for (uint32_t tb = tid; tb < tasks; tb += U * THREADS) {
+                int32_t x[U];
+                uint32_t cl[U], rr[U];
 #pragma unroll
-            for (int u = 0; u < U; u++) {
-                const uint32_t tk = tb + (uint32_t)u * THREADS;
-                const uint32_t c = __umulhi(tk, a.smax_m) >> a.smax_s;
-                const uint32_t r = tk - c * smax;
-                const uint32_t ci = c0 + min(c, nc - 1);
-                const uint32_t cd = ci < (uint32_t)kCandLdsSpec ? L.cand[ci] : gptr(g_cand)[ci - kCandLdsSpec];
-                const uint32_t b = cd >> 26, psid = cd & kPsidMask;
-                const uint32_t spl = L.b_speclen[b];
-                const bool ok = tk < tasks && r < spl;
-                cl[u] = ok ? c : 0xffffffffu;
-                rr[u] = r;
-                sl[u] = spl;
-                x[u] = ok ? (int32_t)gptr(a.buckets[b].ion_bin)[(size_t)psid * spl + r] : 0;
-            }
-#pragma unroll
-            for (int u = 0; u < U; u++) {
-                if (cl[u] == 0xffffffffu) continue;
-                const int32_t lo = x[u] - dF, hi = x[u] + dF;
-                int k = first[(uint32_t)max(lo, 0) >> cs];
-                while (k < P && pk_q[k] < lo) k++;
-                uint32_t n = 0, sum = 0;
-                for (; k < P && pk_q[k] <= hi; k++) {
-                    n++;
-                    sum += (uint32_t)pk_i[k];
+                for (int u = 0; u < U; u++) {
+                    const uint32_t tk = tb + (uint32_t)u * THREADS;
+                    const uint32_t c = __umulhi(tk, dm) >> dsh;
+                    const uint32_t r = tk - c * spl;
+                    const uint32_t ci = s0 + min(c, nrun - 1);
+                    const uint32_t cd = ci < (uint32_t)kCandLdsSpec ? L.cand[ci] : gptr(g_cand)[ci - kCandLdsSpec];
+                    const bool ok = tk < tasks;
+                    // every candidate of the run is of its bucket (a self-check)
+                    n_err += ok && (cd >> 26) != b ? 1u : 0u;
+                    cl[u] = ok ? ci - c0 : 0xffffffffu;
+                    rr[u] = r;
+                    x[u] = ok ? (int32_t)ib[(size_t)(cd & kPsidMask) * spl + r] : 0;
                 }
-                if (n) {
-                    const bool isy = rr[u] >= (sl[u] >> 1);
-                    atomicAdd(&acc_n[cl[u]], isy ? (n << 16) : n);
-                    atomicAdd(isy ? &acc_iy[cl[u]] : &acc_ib[cl[u]], sum);
+#pragma unroll
+                for (int u = 0; u < U; u++) {
+                    if (cl[u] == 0xffffffffu) continue;
+                    const int32_t lo = x[u] - dF, hi = x[u] + dF;
+                    int kk = first[(uint32_t)max(lo, 0) >> cs];
+                    while (kk < P && pk_q[kk] < lo) kk++;
+                    uint32_t n = 0, sum = 0;
+                    for (; kk < P && pk_q[kk] <= hi; kk++) {
+                        n++;
+                        sum += (uint32_t)pk_i[kk];
+                    }
+                    if (n) {
+                        const bool isy = rr[u] >= (spl >> 1);
+                        atomicAdd(&acc_n[cl[u]], isy ? (n << 16) : n);
+                        atomicAdd(isy ? &acc_iy[cl[u]] : &acc_ib[cl[u]], sum);
+                    }
                 }
             }
         }
@@ -571,7 +589,10 @@ __device__ void rescore_candidates(const ItemArgs &a, SpecLds<THREADS, TILE_WORD
         pk_q[k] = 0;
         pk_i[k] = 0;
     }
-    if (tid == 0) L.n_cand = 0;
+    if (tid == 0) {
+        L.n_cand = 0;
+        L.n_seg = 0;
+    }
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) {
         const unsigned long long k2 = __shfl_xor(bkey, o, 64);
@@ -636,6 +657,7 @@ __global__ void __launch_bounds__(THREADS, WAVES_PER_SIMD) search_spectra_kernel
             L.w_shp[wave] = 0;
             L.w_cps[wave] = 0;
         }
+        if (tid == 0) L.n_seg = 0;  // (the unit's candidate list is empty)
         uint32_t n_scanned = 0, n_errors = 0;
         const uint64_t pos_s = a.pos_off[s] - a.pos_off[a.chunk_s0];
         // the spectrum's per-bucket plan, bucket b in lane b of every wave (one
@@ -702,6 +724,14 @@ __global__ void __launch_bounds__(THREADS, WAVES_PER_SIMD) search_spectra_kernel
                 __syncthreads();
                 if (L.n_cand + nent > cand_cap)
                     rescore_candidates<THREADS, TILE_WORDS, SL>(a, L, P, qoff, g_cand, n_errors);
+                // a new run of candidates where the bucket changes (ordered before
+                // pass 1's appends by its barriers, and before pass 2 by the
+                // barrier above its call)
+                if (tid == 0 && (L.n_seg == 0 || L.seg_b[L.n_seg - 1] != (uint8_t)b)) {
+                    L.seg_start[L.n_seg] = L.n_cand;
+                    L.seg_b[L.n_seg] = (uint8_t)b;
+                    L.n_seg++;
+                }
                 // ---- pass 1: shared-peak counts of every entry in the tile ----
                 for (int c0 = 0; c0 < nslb; c0 += SL) {
                     const int ns = min(SL, nslb - c0);
