@@ -522,16 +522,31 @@ __device__ void rescore_candidates(const ItemArgs &a, SpecLds<THREADS, TILE_WORD
             const auto ib = gptr(bk.ion_bin);
             const uint32_t nrun = s1 - s0, tasks = nrun * spl;  // < 2^31: the division below is exact
             constexpr int U = 4;
+            // the candidate words of the next iteration's tasks are loaded one
+            // iteration ahead (most sit in global scratch when a spectrum has many)
+            auto cand_of = [&](uint32_t tk) {
+                const uint32_t ci = s0 + min(__umulhi(tk, dm) >> dsh, nrun - 1);
+                return ci < (uint32_t)kCandLdsSpec ? L.cand[ci] : gptr(g_cand)[ci - kCandLdsSpec];
+            };
+            uint32_t cdn[U];
+#pragma unroll
+            for (int u = 0; u < U; u++) cdn[u] = cand_of((uint32_t)tid + (uint32_t)u * THREADS);
             for (uint32_t tb = tid; tb < tasks; tb += U * THREADS) {
                 int32_t x[U];
-                uint32_t cl[U], rr[U];
+                uint32_t cl[U], rr[U], cdc[U];
+#pragma unroll
+                for (int u = 0; u < U; u++) cdc[u] = cdn[u];
+                if (tb + U * THREADS < tasks) {
+#pragma unroll
+                    for (int u = 0; u < U; u++) cdn[u] = cand_of(tb + (uint32_t)(U + u) * THREADS);
+                }
 #pragma unroll
                 for (int u = 0; u < U; u++) {
                     const uint32_t tk = tb + (uint32_t)u * THREADS;
                     const uint32_t c = __umulhi(tk, dm) >> dsh;
                     const uint32_t r = tk - c * spl;
                     const uint32_t ci = s0 + min(c, nrun - 1);
-                    const uint32_t cd = ci < (uint32_t)kCandLdsSpec ? L.cand[ci] : gptr(g_cand)[ci - kCandLdsSpec];
+                    const uint32_t cd = cdc[u];
                     const bool ok = tk < tasks;
                     // every candidate of the run is of its bucket (a self-check)
                     n_err += ok && (cd >> 26) != b ? 1u : 0u;
