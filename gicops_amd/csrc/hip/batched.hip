@@ -734,11 +734,15 @@ __global__ void __launch_bounds__(THREADS, WAVES_PER_SIMD) search_spectra_kernel
                 const uint32_t nent = (uint32_t)(thi - tlo + 1);
                 const uint32_t tlo_id = (uint32_t)tlo * speclen;
                 const uint32_t span_id = nent * speclen;
-                // room for every entry of this tile as a candidate (a full barrier:
-                // pass 2 reads candidates that pass 1 wrote to global scratch)
-                __syncthreads();
-                if (L.n_cand + nent > cand_cap)
+                // room for every entry of this tile as a candidate? An LDS-only
+                // barrier, so that the next step's slice words loaded by the last
+                // step stay in flight; a full one before pass 2, which reads
+                // candidates that pass 1 wrote to global scratch
+                lds_barrier();
+                if (L.n_cand + nent > cand_cap) {
+                    __syncthreads();
                     rescore_candidates<THREADS, TILE_WORDS, SL>(a, L, P, qoff, g_cand, n_errors);
+                }
                 // a new run of candidates where the bucket changes (ordered before
                 // pass 1's appends by its barriers, and before pass 2 by the
                 // barrier above its call)
