@@ -185,7 +185,9 @@ def main() -> int:
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="open2m", choices=["open2m", "uneven", "small", "c4"])
+    ap.add_argument("--workload", default="open2m", choices=["open2m", "uneven", "small", "c4"],
+                    help="c4 is 1M spectra of the uneven mix on the 50M-peptide index; run it with --steps 2 "
+                         "--warmup 0 --evalue plain --no-compare-spectrum --config 0 (about 2 minutes)")
     ap.add_argument("--spectra", type=int, default=0, help="override the batch size")
     ap.add_argument("--cpu-budget", type=float, default=15.0, help="seconds of CPU baseline work")
     ap.add_argument("--no-cpu", action="store_true")
