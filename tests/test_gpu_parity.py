@@ -2,7 +2,7 @@
 
 Bar: shared-peak counts, candidate counts, histograms and top-hit ids are
 bit-exact; hyperscores within 1e-5 relative; e-values within 1e-3 relative of
-log10 (north-star tolerances).
+log10.
 """
 import sys
 
