@@ -15,7 +15,7 @@ pytestmark = pytest.mark.gpu
 
 def test_cpp_dslim_queryspectrum_tsv(small_index, small_batch, tmp_path):
     print("case: C++ adapter end to end, small workload", flush=True)
-    # the driver builds the same workload from the same seeds as the presets
+    # the adapter_e2e program builds the same workload from the same seeds as the presets
     assert slm.DB_PRESETS["small"].n_peptides == 50_000 and slm.DB_PRESETS["small"].seed == 0
     sc = slm.SPEC_PRESETS["small"]
     assert (sc.n_spectra, sc.seed, sc.noise_min, sc.noise_max, sc.qalen) == (1000, 0, 20, 80, 100)
