@@ -25,11 +25,18 @@ from __future__ import annotations
 import argparse
 import json
 
-import numpy as np
 import os
+import statistics
+import subprocess
 import sys
 import time
 from pathlib import Path
+
+# The CPU baseline's OpenMP threads are bound to consecutive cores. libgomp
+# reads this when it loads (with the host library, before any search).
+os.environ.setdefault("OMP_PROC_BIND", "close")
+
+import numpy as np  # noqa: E402
 
 ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT))
@@ -74,11 +81,28 @@ def traffic_from_profiles(workload: str, config: int, whole_bin: int):
     return float(vals[key]) if key in vals else None
 
 
-def cpu_baseline(index, batch, params_ffi, budget_s: float):
-    """The oracle (CPU restatement of DSLIM_QuerySpectrum + ModelTailFit) on a
-    bounded sample of the same batch, all host cores."""
-    import numpy as np
+def cpu_topology() -> dict:
+    """Sockets x cores x SMT of the host (lscpu), and the CPUs this process may use."""
+    topo = {"cpus_usable": len(os.sched_getaffinity(0)), "nproc": os.cpu_count()}
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=10).stdout
+        keys = {"Socket(s)": "sockets", "Core(s) per socket": "cores_per_socket", "Thread(s) per core": "smt",
+                "Model name": "model"}
+        for line in out.splitlines():
+            k, _, v = line.partition(":")
+            if k.strip() in keys:
+                v = v.strip()
+                topo[keys[k.strip()]] = int(v) if v.isdigit() else v
+    except (OSError, subprocess.SubprocessError):
+        pass
+    return topo
 
+
+def cpu_baseline(index, batch, params_ffi, budget_s: float, repeats: int = 3):
+    """The oracle (the CPU restatement of DSLIM_QuerySpectrum + ModelTailFit,
+    dslim_query.cpp:749-1081 and expeRT.cpp:396-559) on a bounded sample of
+    the same batch: OpenMP over spectra on every usable host CPU, threads
+    bound close (OMP_PROC_BIND), median of `repeats` timed runs."""
     from oracle import pyoracle as O
 
     cores = len(os.sched_getaffinity(0))
@@ -90,11 +114,17 @@ def cpu_baseline(index, batch, params_ffi, budget_s: float):
     rate = probe.n_spectra / dt
     m = int(min(n, max(probe.n_spectra, rate * budget_s)))
     sample = batch.subset(np.arange(m))
-    t = time.perf_counter()
-    O.search(index, sample, params_ffi, threads=cores)
-    dt = time.perf_counter() - t
-    return {"value": round(m / dt, 2), "unit": "spectra/s", "cores": cores, "kind": "port",
-            "sample": f"first {m} spectra of the same batch, OpenMP over spectra, {dt:.1f} s"}
+    rates = []
+    for _ in range(repeats):
+        t = time.perf_counter()
+        O.search(index, sample, params_ffi, threads=cores)
+        rates.append(m / (time.perf_counter() - t))
+    return {"value": round(statistics.median(rates), 2), "unit": "spectra/s", "cores": cores, "kind": "port",
+            "port_of": "dslim_query.cpp:749-1081 (DSLIM_QuerySpectrum) + expeRT.cpp:396-559 (ModelTailFit)",
+            "runs_spectra_per_s": [round(r, 2) for r in rates],
+            "topology": cpu_topology(), "omp_proc_bind": os.environ.get("OMP_PROC_BIND"),
+            "sample": f"first {m} spectra of the same batch, OpenMP over spectra (schedule dynamic), "
+                      f"median of {repeats} runs"}
 
 
 def train_surrogate(s, index, sc, dM: float, args) -> dict:
@@ -189,7 +219,7 @@ def main() -> int:
                     help="c4 is 1M spectra of the uneven mix on the 50M-peptide index; run it with --steps 2 "
                          "--warmup 0 --evalue plain --no-compare-spectrum --config 0 (about 2 minutes)")
     ap.add_argument("--spectra", type=int, default=0, help="override the batch size")
-    ap.add_argument("--cpu-budget", type=float, default=15.0, help="seconds of CPU baseline work")
+    ap.add_argument("--cpu-budget", type=float, default=8.0, help="seconds of CPU baseline work per run (3 runs)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--engine", default="batched", choices=["batched", "spectrum"])
     ap.add_argument("--config", type=int, default=-1, help="batched launch configuration (-1: try all, keep the best)")
@@ -202,6 +232,8 @@ def main() -> int:
     ap.add_argument("--surrogate-conf", type=float, default=0.9)
     ap.add_argument("--whole-bin", type=int, default=0, help="whole-bin threshold (0: tuned)")
     ap.add_argument("--no-graph", action="store_true", help="time eager launches instead of hipGraph replays")
+    ap.add_argument("--no-bandwidth", action="store_true",
+                    help="skip the read-bandwidth probe (for short profiler runs)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run a device-to-device copy of MIB MiB (HBM counter calibration) and exit")
     args = ap.parse_args()
@@ -264,6 +296,15 @@ def main() -> int:
         f"evalue {st['evalue_ms']:.1f}), {st['items']} items, {st['chunks']} chunks, config {st['config']}")
     eager = dict(st)
     judge, _ = s.download(dq)
+    if engine == F.GX_ENGINE_BATCHED:
+        # a new batch takes the eager path: plan on the device, host scheduling
+        # of the work units, then the launches (buffers already sized)
+        t = time.perf_counter()
+        st2 = s.search_device(dq, plain)
+        extra["new_batch_ms_per_step"] = round((time.perf_counter() - t) * 1000.0, 3)
+        extra["new_batch_device_ms"] = round(st2["total_ms"], 3)
+        log(f"eager plain again (a new batch's cost): {extra['new_batch_ms_per_step']:.1f} ms wall, "
+            f"{st2['total_ms']:.1f} ms device")
     use_graph = engine == F.GX_ENGINE_BATCHED and not args.no_graph
     learned = args.evalue == "learned" and engine == F.GX_ENGINE_BATCHED
     if learned:
@@ -317,6 +358,8 @@ def main() -> int:
 
     measured = {}
     try:
+        if args.no_bandwidth:
+            raise RuntimeError("skipped (--no-bandwidth)")
         measured = {"sequential": round(s.read_bandwidth(0), 1), "random_1x128B": round(s.read_bandwidth(1), 1),
                     "random_2x128B": round(s.read_bandwidth(2), 1)}
         log(f"measured read bandwidth: {measured}")

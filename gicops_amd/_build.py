@@ -60,14 +60,26 @@ def build_host() -> Path:
     return out
 
 
+HIP_FLAGS = [f"--offload-arch={OFFLOAD_ARCH}", "-O3", "-std=c++17", "-fPIC", "-ffp-contract=off",
+             "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+             "-Wall", "-Wno-unused-parameter", "-Wno-unused-command-line-argument"]
+
+
 def build_hip() -> Path:
+    """Each source compiled on its own (in parallel), then one link."""
+    from concurrent.futures import ThreadPoolExecutor
+
     LIBDIR.mkdir(parents=True, exist_ok=True)
+    objdir = ROOT / "_build" / "hip"
+    objdir.mkdir(parents=True, exist_ok=True)
     out = LIBDIR / "libgicops_hip.so"
     tmp = out.with_suffix(".so.tmp")
-    _run([HIPCC, f"--offload-arch={OFFLOAD_ARCH}", "-O3", "-std=c++17", "-fPIC", "-shared",
-          "-ffp-contract=off", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
-          "-Wall", "-Wno-unused-parameter", "-Wno-unused-command-line-argument",
-          f"-I{INCLUDE}", f"-I{CSRC / 'hip'}", *HIP_SOURCES, "-o", tmp])
+    objs = [objdir / (src.stem + ".o") for src in HIP_SOURCES]
+    jobs = max(1, min(len(HIP_SOURCES), int(os.environ.get("MAX_JOBS", os.cpu_count() or 1))))
+    with ThreadPoolExecutor(jobs) as ex:
+        list(ex.map(lambda so: _run([HIPCC, *HIP_FLAGS, f"-I{INCLUDE}", f"-I{CSRC / 'hip'}", "-c", so[0],
+                                     "-o", so[1]]), zip(HIP_SOURCES, objs)))
+    _run([HIPCC, f"--offload-arch={OFFLOAD_ARCH}", "-shared", "-fPIC", *objs, "-o", tmp])
     os.replace(tmp, out)
     return out
 

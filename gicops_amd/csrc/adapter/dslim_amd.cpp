@@ -17,7 +17,7 @@ struct State {
     gx_dindex *didx = nullptr;
     std::vector<std::vector<float>> masses;  // contiguous copies of pepEntries[].Mass
     std::ostream *out = nullptr;
-    std::string query_file;
+    std::vector<std::string> query_files;  // indexed by hCell::fileIndex
     std::mutex mu;
 };
 
@@ -122,11 +122,15 @@ status_t DSLIM_QuerySpectrum(Queries<int> *ss, Index *index, uint_t idxchunk, in
 }
 
 status_t DFile_InitFiles(std::ostream *out, const std::string &query_file) {
+    return DFile_InitFiles(out, std::vector<std::string>{query_file});
+}
+
+status_t DFile_InitFiles(std::ostream *out, const std::vector<std::string> &query_files) {
     State &S = state();
     if (!out) return GX_ERR_INVLD_PTR;
     std::lock_guard<std::mutex> lk(S.mu);
     S.out = out;
-    S.query_file = query_file;
+    S.query_files = query_files;
     *out << "file\t" << "scan_num\t" << "prec_mass\t" << "charge\t" << "retention_time\t" << "peptide\t"
          << "matched_ions\t" << "total_ions\t" << "calc_pep_mass\t" << "mass_diff\t" << "mod_info\t"
          << "hyperscore\t" << "expectscore\t" << "num_hits" << std::endl;
@@ -142,8 +146,9 @@ status_t DFile_PrintScore(Index *index, uint_t specid, float pmass, hCell *psm, 
     const pepEntry &e = lcl->pepEntries[psm->psid];
     const std::string pep(lcl->pepIndex.seqs + (size_t)e.seqID * peplen, peplen);
     std::lock_guard<std::mutex> lk(S.mu);
+    if (psm->fileIndex >= S.query_files.size()) return GX_ERR_INVLD_PARAM;
     std::ostream &o = *S.out;
-    o << S.query_file << '\t' << std::to_string(specid + 1) << '\t' << std::to_string(pmass) << '\t'
+    o << S.query_files[psm->fileIndex] << '\t' << std::to_string(specid + 1) << '\t' << std::to_string(pmass) << '\t'
       << std::to_string(psm->pchg) << '\t' << std::to_string(psm->rtime) << '\t' << pep << '\t'
       << std::to_string(psm->sharedions) << '\t' << std::to_string(psm->totalions) << '\t'
       << std::to_string(e.Mass) << '\t' << std::to_string(pmass - e.Mass) << '\t' << '\t'

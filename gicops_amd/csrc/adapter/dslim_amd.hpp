@@ -17,6 +17,7 @@
 #include <cstdint>
 #include <ostream>
 #include <string>
+#include <vector>
 
 #include "gicops_amd.h"
 
@@ -112,6 +113,10 @@ status_t DSLIM_GPU_Deinitialize();
 status_t DSLIM_QuerySpectrum(Queries<int> *ss, Index *index, uint_t idxchunk, int currSpecID);
 
 // TSV output (one stream; the reference writes one file per OpenMP thread).
+// `query_files` are the names the reference keeps in its global `queryfiles`
+// (dslim_fileout.cpp:25,128): a row prints query_files[psm->fileIndex].
+status_t DFile_InitFiles(std::ostream *out, const std::vector<std::string> &query_files);
+// one query file: the same as a list of one name
 status_t DFile_InitFiles(std::ostream *out, const std::string &query_file);
 status_t DFile_PrintScore(Index *index, uint_t specid, float pmass, hCell *psm, double e_x, uint_t npsms);
 status_t DFile_DeinitFiles();

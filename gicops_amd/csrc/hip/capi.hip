@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstring>
 #include <mutex>
@@ -167,6 +168,10 @@ namespace gx {
 
 // A device index from bucket metadata. bA/iA come from the host buckets, or,
 // when dev_bA/dev_iA are given, are device arrays the index takes over.
+// generations of device indexes: a captured graph replays only on the index
+// it was captured on, recognised by (address, generation)
+std::atomic<uint64_t> g_index_gen{0};
+
 int index_create(gx_ctx *c, const gx_bucket *b, int nb, int maxz, uint32_t *const *dev_bA, uint32_t *const *dev_iA,
                  gx_dindex **out) {
     const bool on_dev = dev_bA && dev_iA;
@@ -174,6 +179,7 @@ int index_create(gx_ctx *c, const gx_bucket *b, int nb, int maxz, uint32_t *cons
     gx_dindex *d = new (std::nothrow) gx_dindex();
     if (!d) return GX_ERR_BAD_MEM_ALLOC;
     d->ctx = c;
+    d->gen = ++g_index_gen;
     d->n_buckets = nb;
     d->maxz = maxz;
     d->h_buckets.resize(nb);
@@ -643,7 +649,7 @@ int search_batched(gx_ctx *c, const gx_dindex *idx, gx_dqueries *q, const gx_sea
 
     // ---- replay a captured graph of the same batch and parameters ----
     if (p->use_graph && q->graph_exec && q->graph_key == key && q->graph_ctx_gen == c->scratch_gen &&
-        q->graph_buf_gen == q->buf_gen) {
+        q->graph_buf_gen == q->buf_gen && q->graph_index == idx && q->graph_index_gen == idx->gen) {
         GX_HIP(hipEventRecord(c->ev0, sm));
         GX_HIP(hipGraphLaunch(q->graph_exec, sm));
         GX_HIP(hipEventRecord(c->ev3, sm));
@@ -926,6 +932,8 @@ int search_batched(gx_ctx *c, const gx_dindex *idx, gx_dqueries *q, const gx_sea
         q->graph_key = key;
         q->graph_ctx_gen = c->scratch_gen;
         q->graph_buf_gen = q->buf_gen;
+        q->graph_index = idx;
+        q->graph_index_gen = idx->gen;
     }
     return GX_SUCCESS;
 }

@@ -45,6 +45,7 @@ struct gx_ctx {
 
 struct gx_dindex {
     gx_ctx *ctx = nullptr;
+    uint64_t gen = 0;  // unique per index created in this process (never reused, unlike the address)
     int n_buckets = 0;
     int maxz = 1;
     uint32_t nbins = 0;
@@ -112,6 +113,8 @@ struct gx_dqueries {
     std::vector<char> graph_key;
     uint64_t buf_gen = 0;  // bumped whenever a chunk buffer moves
     uint64_t graph_ctx_gen = 0, graph_buf_gen = 0;
+    const gx_dindex *graph_index = nullptr;  // the index the graph was captured on
+    uint64_t graph_index_gen = 0;             // and its generation: a freed index's address may come back
     // learned e-value stage: features, targets, activations, routing
     bf16 *l_feat = nullptr, *l_h1 = nullptr, *l_h2 = nullptr;
     float *l_anchor = nullptr, *l_tgt = nullptr, *l_loge = nullptr, *l_out = nullptr;

@@ -88,6 +88,7 @@ int main(int argc, char **argv) {
     q.rtimes = gq->rtimes;
     q.numSpecs = (int)gq->n_spectra;
     q.numPeaks = (int)gq->n_peaks;
+    q.fileNum = 1;  // the second query file: rows must print its name, not the first one's
 
     params.dM = 10.0;
     params.expect_max = 20.0;
@@ -97,7 +98,7 @@ int main(int argc, char **argv) {
         return 6;
     }
     std::ofstream out(argv[1]);
-    DFile_InitFiles(&out, "query.ms2");
+    DFile_InitFiles(&out, std::vector<std::string>{"other.ms2", "query.ms2"});
     st = DSLIM_QuerySpectrum(&q, index.data(), (uint_t)nb, 0);
     DFile_DeinitFiles();
     DSLIM_GPU_Deinitialize();

@@ -118,3 +118,45 @@ def test_partials_error_codes(small_index):
         F.GX_ERR_INVLD_PTR, F.GX_ERR_INVLD_PARAM)
     dq.free()
     sh.close()
+
+
+def test_graph_replay_follows_the_index(small_index):
+    """A batch searched with use_graph on two live shard indexes in turn: each
+    replay must equal that shard's own eager result (a graph captured on one
+    index is never replayed on another, also after the first is freed and a
+    new index may take its address)."""
+    say("case: graph replays on two shard indexes")
+    batch = slm.generate_spectra(small_index, slm.SPEC_PRESETS["small"])
+    sh = ShardedSearcher(small_index, 2)
+    lib = sh._lib
+    dq = sh.upload(batch)
+    n = batch.n_spectra
+
+    def run(d, graph):
+        pf = SearchParams(dM=100.0, use_graph=graph).to_ffi()
+        st = F.gx_search_stats()
+        F.check("gx_search_device", lib.gx_search_device(sh._ctx, d, dq._h, C.byref(pf), C.byref(st)))
+        psm = np.zeros(n, dtype=F.PSM_DTYPE)
+        F.check("gx_results_download", lib.gx_results_download(sh._ctx, dq._h, F.ptr(psm), None))
+        return psm
+
+    eager = [run(d, False) for d in sh._didx]
+    assert eager[0].tobytes() != eager[1].tobytes()
+    for rnd in range(2):
+        for k, d in enumerate(sh._didx):
+            for rep in range(2):  # capture (or re-capture) on this index, then replay
+                got = run(d, True)
+                assert got.tobytes() == eager[k].tobytes(), (rnd, k, rep)
+    # free shard 1 and build a new index in its place: the batch's graph was
+    # last captured on shard 1, and the new index may reuse its address
+    lib.gx_index_free(sh._didx[1])
+    bp = small_index.build_params()
+    d2 = C.c_void_p()
+    F.check("gx_index_build", lib.gx_index_build(sh._ctx, small_index.raw_array, len(small_index.buckets),
+                                                 C.byref(bp), C.byref(d2)))
+    sh._didx[1] = d2
+    whole = run(d2, False)
+    assert run(d2, True).tobytes() == whole.tobytes()
+    assert run(d2, True).tobytes() == whole.tobytes()
+    dq.free()
+    sh.close()
