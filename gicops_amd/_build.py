@@ -40,6 +40,7 @@ HIP_SOURCES = [
     CSRC / "hip" / "preprocess.hip",
     CSRC / "hip" / "partials.hip",
     CSRC / "hip" / "bandwidth.hip",
+    CSRC / "hip" / "noalloc.hip",
 ]
 HIP_HEADERS = sorted((CSRC / "hip").glob("*.h")) + sorted(INCLUDE.glob("*.h"))
 

@@ -287,6 +287,8 @@ def hip_lib() -> C.CDLL:
                                           C.POINTER(gx_search_params), C.c_int, vp, vp, C.POINTER(C.c_float)]
         lib.gx_search_device.argtypes = [vp, vp, vp, C.POINTER(gx_search_params), C.POINTER(gx_search_stats)]
         lib.gx_results_download.argtypes = [vp, vp, vp, vp]
+        lib.gx_search_spectra_noalloc.argtypes = [vp, vp, C.POINTER(gx_query_batch), C.POINTER(gx_search_params),
+                                                  vp, vp, C.POINTER(gx_search_stats)]
         lib.gx_search.argtypes = [vp, vp, C.POINTER(gx_query_batch), C.POINTER(gx_search_params), vp, vp,
                                   C.POINTER(gx_search_stats)]
         lib.gx_ctx_device_info.argtypes = [vp, C.c_char_p, C.c_size_t, C.POINTER(C.c_int)]

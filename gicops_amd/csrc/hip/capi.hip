@@ -113,6 +113,7 @@ extern "C" int gx_ctx_create(int device, gx_ctx **out) {
     c->n_cu = prop.multiProcessorCount;
     std::snprintf(c->name, sizeof(c->name), "%s (%s)", prop.name, prop.gcnArchName);
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) return fail(GX_ERR_DEVICE);
+    if (int st = noalloc_setup(device)) return fail(st);
     if (hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess ||
         hipEventCreate(&c->ev2) != hipSuccess || hipEventCreate(&c->ev3) != hipSuccess)
         return fail(GX_ERR_DEVICE);

@@ -144,6 +144,10 @@ namespace gx {
 int index_create(gx_ctx *c, const gx_bucket *b, int nb, int maxz, uint32_t *const *dev_bA, uint32_t *const *dev_iA,
                  gx_dindex **out);
 
+// the no-alloc search's static device scratch: its address on `device`
+// (the current device), looked up once per device (noalloc.hip)
+int noalloc_setup(int device);
+
 // learned e-value stage of a batched search (mlp_capi.hip)
 int ensure_learned(gx_ctx *c, gx_dqueries *q);
 hipError_t enqueue_learned_evalue(gx_ctx *c, gx_dqueries *q, EvalArgs ea, float conf, hipStream_t s);

@@ -247,6 +247,25 @@ int gx_results_download(gx_ctx *ctx, gx_dqueries *q, gx_psm *psms, uint32_t *his
 int gx_search(gx_ctx *ctx, const gx_dindex *idx, const gx_query_batch *q, const gx_search_params *p,
               gx_psm *psms, uint32_t *hist, gx_search_stats *stats);
 
+/* The per-spectrum engine's search (GX_ENGINE_SPECTRUM, plain tail fit)
+ * without any device allocation. No counterpart in the reference: it is for
+ * callers that cannot allocate device memory (code under HIP graph capture
+ * elsewhere in the process, tight serving loops, harnesses that forbid
+ * hipMalloc). Every pointer of `q` (idx, moz, intensity, precurse, dm) and
+ * `psms` (n_spectra records) / `hist` (NULL or n_spectra*GX_HIST_BINS u32)
+ * are device memory of the context's GPU, owned by the caller. The work
+ * buffers live in one static device array of 60 MiB per GPU, held until the
+ * process ends: the per-block slice and candidate overflow of up to 512
+ * blocks. Limits: at most 2048 peaks per spectrum and peaks x (2 dF + 1) <=
+ * 4096 (GX_ERR_INVLD_SIZE beyond; a malformed idx array is GX_ERR_INVLD_SIZE
+ * too). The array keeps the previous call's contents; each call clears the
+ * parts that must start at zero. All calls on one GPU share it, so the
+ * library serialises them per device within the process (calls from other
+ * threads wait); the call returns when the search is complete. Results equal
+ * gx_search_device's with engine GX_ENGINE_SPECTRUM. */
+int gx_search_spectra_noalloc(gx_ctx *ctx, const gx_dindex *idx, const gx_query_batch *q,
+                              const gx_search_params *p, gx_psm *psms, uint32_t *hist, gx_search_stats *stats);
+
 /* Autotuner result: the launch configuration and window split chosen, and
  * the device time of every candidate tried on the sample batch. */
 #define GX_TUNE_MAX 16
