@@ -36,6 +36,7 @@ HIP_SOURCES = [
     CSRC / "hip" / "capi.hip",
     CSRC / "hip" / "mlp.hip",
     CSRC / "hip" / "mlp_capi.hip",
+    CSRC / "hip" / "learned.hip",
     CSRC / "hip" / "index_build.hip",
     CSRC / "hip" / "preprocess.hip",
     CSRC / "hip" / "partials.hip",

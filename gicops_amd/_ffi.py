@@ -106,21 +106,21 @@ class gx_tune_result(C.Structure):
 
 
 GX_TRAIN_MAX_EPOCHS = 64
-MLP_FEAT, MLP_HIDDEN, MLP_OUT_PAD, MLP_OUT = 128, 256, 16, 3
+MLP_FEAT, MLP_HIDDEN, MLP_OUT_PAD, MLP_OUT, MLP_TARGET = 128, 256, 16, 3, 4
 
 
 class gx_train_params(C.Structure):
     _fields_ = [
         ("batch", C.c_int32), ("epochs", C.c_int32), ("lr", C.c_float), ("lr_decay", C.c_float),
-        ("conf_weight", C.c_float), ("tol", C.c_float), ("seed", C.c_uint64),
+        ("conf_weight", C.c_float), ("pad", C.c_int32), ("seed", C.c_uint64),
     ]
 
 
 class gx_train_report(C.Structure):
     _fields_ = [
         ("samples", C.c_uint64), ("epochs", C.c_int32), ("pad", C.c_int32),
-        ("mse_slope", C.c_float * GX_TRAIN_MAX_EPOCHS), ("mse_u", C.c_float * GX_TRAIN_MAX_EPOCHS),
-        ("bce", C.c_float * GX_TRAIN_MAX_EPOCHS), ("within_tol", C.c_float * GX_TRAIN_MAX_EPOCHS),
+        ("mse_mark", C.c_float * GX_TRAIN_MAX_EPOCHS), ("mse_mark2", C.c_float * GX_TRAIN_MAX_EPOCHS),
+        ("bce", C.c_float * GX_TRAIN_MAX_EPOCHS), ("in_window", C.c_float * GX_TRAIN_MAX_EPOCHS),
     ]
 
 
@@ -304,9 +304,10 @@ def hip_lib() -> C.CDLL:
         lib.gx_mlp_get_params.argtypes = [vp, fp, C.c_size_t, fp]
         lib.gx_mlp_add_training.argtypes = [vp, vp, C.POINTER(C.c_uint64)]
         lib.gx_mlp_train.argtypes = [vp, C.POINTER(gx_train_params), C.POINTER(gx_train_report)]
-        lib.gx_mlp_grad.argtypes = [vp, fp, fp, fp, C.c_uint32, fp, C.c_float, C.c_float, fp, fp, fp]
+        lib.gx_mlp_grad.argtypes = [vp, fp, fp, C.c_uint32, fp, C.c_float, fp, fp, fp]
         lib.gx_mlp_predict.argtypes = [vp, fp, C.c_uint32, fp]
         lib.gx_ctx_set_surrogate.argtypes = [vp, vp]
-        lib.gx_features_download.argtypes = [vp, vp, fp, fp, fp, fp, u8p, u8p]
+        lib.gx_features_download.argtypes = [vp, vp, fp, fp, fp, u8p, u8p]
+        lib.gx_surrogate_outputs_download.argtypes = [vp, vp, fp]
         _HIP = lib
     return _HIP

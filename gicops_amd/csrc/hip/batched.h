@@ -123,8 +123,6 @@ struct EvalArgs {
     const uint32_t *best_shp;
     const uint32_t *hist;
     const uint32_t *cpsms;
-    const uint8_t *route;    // mode 2: 1 = surrogate result already written
-    int mode;                // 0: top hit + fit; 1: top hit only; 2: fit where route == 0
     gx_psm *out;
     unsigned long long *stats;
 };

@@ -974,7 +974,8 @@ __global__ void __launch_bounds__(64) fold_kernel(FoldArgs a) {
 }
 
 // ---------------------------------------------------------------------------
-// one 64-thread block per spectrum
+// one 64-thread block per spectrum: the top hit, the plain tail fit and the
+// e-value (dslim_query.cpp:1017-1061, expeRT.cpp:396-559)
 __global__ void __launch_bounds__(64) evalue_kernel(EvalArgs a) {
     __shared__ uint32_t hist[GX_HIST_BINS];
     __shared__ double sx[GX_HIST_BINS];
@@ -982,43 +983,10 @@ __global__ void __launch_bounds__(64) evalue_kernel(EvalArgs a) {
     const int lane = threadIdx.x;
     const uint32_t s = blockIdx.x;
     const uint32_t need = (uint32_t)max(a.min_cpsm, 1);
-    gx_psm r;
-    if (a.mode == 2) {
-        // the surrogate's fallback: fit the modeled spectra it did not accept
-        r = a.out[s];
-        if (a.route[s] || !(r.flags & GX_PSM_MODELED)) return;
-    } else {
-        const unsigned long long key = a.best_key[s];
-        const uint32_t eq = a.best_eq[s], shp = a.best_shp[s];
-        const uint32_t cps = a.cpsms[s];
-        r.hyperscore = 0.0f;
-        r.psid = -1;
-        r.idxoffset = 0;
-        r.sharedions = 0;
-        r.totalions = 0;
-        r.flags = 0;
-        r.cpsms = (int32_t)cps;
-        r.hyp = 0;
-        r.mu = 0;
-        r.beta = 0;
-        r.evalue = INFINITY;
-        if (cps >= 1 && key) {
-            const uint32_t low = ~(uint32_t)(key & 0xffffffffu);
-            const uint32_t b = low >> 26, psid = low & ((1u << 26) - 1);
-            r.flags |= GX_PSM_HAS_HIT;
-            r.hyperscore = __uint_as_float((uint32_t)(key >> 32));
-            r.psid = (int32_t)psid;
-            r.idxoffset = (uint16_t)b;
-            r.sharedions = (uint16_t)shp;
-            r.totalions = (uint16_t)a.buckets[b].speclen;
-            r.hyp = hyp_bin(r.hyperscore);
-            if (eq > 1) r.flags |= GX_PSM_TIE;
-        }
-        if (cps >= need && r.psid >= 0) r.flags |= GX_PSM_MODELED;
-        if (a.mode == 1 || !(r.flags & GX_PSM_MODELED)) {
-            if (lane == 0) a.out[s] = r;
-            return;
-        }
+    gx_psm r = top_hit_psm(a.best_key[s], a.best_eq[s], a.best_shp[s], a.cpsms[s], a.buckets, need);
+    if (!(r.flags & GX_PSM_MODELED)) {
+        if (lane == 0) a.out[s] = r;
+        return;
     }
     for (int w = lane; w < GX_HIST_BINS; w += 64) hist[w] = a.hist[(size_t)s * GX_HIST_BINS + w];
     __syncthreads();

@@ -331,8 +331,8 @@ extern "C" void gx_queries_free(gx_dqueries *d) {
                   d->pair_tiles, d->pair_pack, d->pair_pos_off, d->pair_nsl, d->n_items,     d->item_off, d->n_pos,
                   d->pos_off,   d->cpsms,      d->counters,      d->best_key,     d->best_eq,  d->best_shp,
                   d->pos,       d->unit_s,     d->unit_br,       d->fold_s,       d->fold_p0,  d->fold_n,
-                  d->part_key,  d->part_eq,    d->part_shp,      d->part_cps,     d->part_hist, d->porder,           d->l_feat,       d->l_h1,     d->l_h2,
-                  d->l_anchor,  d->l_tgt,      d->l_loge,     d->l_out,         d->l_valid,      d->l_sparse, d->l_route};
+                  d->part_key,  d->part_eq,    d->part_shp,      d->part_cps,     d->part_hist, d->porder,           d->l_feat,
+                  d->l_tgt,      d->l_loge,     d->l_out,         d->l_valid,      d->l_serv};
     for (void *p : ps)
         if (p) (void)hipFree(p);
     delete d;
@@ -412,6 +412,7 @@ int search_plain(gx_ctx *c, const gx_dindex *idx, gx_dqueries *q, const gx_searc
     GX_HIP(hipEventRecord(c->ev1, c->stream));
     GX_HIP(hipStreamSynchronize(c->stream));
     q->searched = true;
+    q->learned_done = false;
     q->batched_state = false;
     q->searched_index = idx;
     if (stats) {
@@ -569,8 +570,6 @@ void fill_item_args(Batched &B, gx_ctx *c, const gx_dindex *idx, gx_dqueries *q,
     ea.best_shp = q->best_shp;
     ea.hist = q->hist;
     ea.cpsms = q->cpsms;
-    ea.route = nullptr;
-    ea.mode = 0;
     ea.out = q->psm;
     ea.stats = c->d_stats;
 }
@@ -656,6 +655,7 @@ int search_batched(gx_ctx *c, const gx_dindex *idx, gx_dqueries *q, const gx_sea
         GX_HIP(hipEventRecord(c->ev3, sm));
         GX_HIP(hipStreamSynchronize(sm));
         q->searched = true;
+        q->learned_done = B.learned;
         q->batched_state = true;
         q->searched_index = idx;
         if (stats) {
@@ -882,6 +882,7 @@ int search_batched(gx_ctx *c, const gx_dindex *idx, gx_dqueries *q, const gx_sea
     GX_HIP(hipEventRecord(c->ev3, sm));
     GX_HIP(hipStreamSynchronize(sm));
     q->searched = true;
+    q->learned_done = B.learned;
     q->batched_state = true;
     q->searched_index = idx;
     if (stats) {

@@ -115,10 +115,12 @@ struct gx_dqueries {
     uint64_t graph_ctx_gen = 0, graph_buf_gen = 0;
     const gx_dindex *graph_index = nullptr;  // the index the graph was captured on
     uint64_t graph_index_gen = 0;             // and its generation: a freed index's address may come back
-    // learned e-value stage: features, targets, activations, routing
-    bf16 *l_feat = nullptr, *l_h1 = nullptr, *l_h2 = nullptr;
-    float *l_anchor = nullptr, *l_tgt = nullptr, *l_loge = nullptr, *l_out = nullptr;
-    uint8_t *l_valid = nullptr, *l_sparse = nullptr, *l_route = nullptr;
+    // learned e-value stage: training features and targets; the fused
+    // stage's raw outputs and served flags (n x 4)
+    bf16 *l_feat = nullptr;
+    float *l_tgt = nullptr, *l_loge = nullptr, *l_out = nullptr;
+    uint8_t *l_valid = nullptr, *l_serv = nullptr;
+    bool learned_done = false;  // the last search ran the learned stage (l_out holds its outputs)
 };
 
 namespace gx {
@@ -150,7 +152,7 @@ int noalloc_setup(int device);
 
 // learned e-value stage of a batched search (mlp_capi.hip)
 int ensure_learned(gx_ctx *c, gx_dqueries *q);
-hipError_t enqueue_learned_evalue(gx_ctx *c, gx_dqueries *q, EvalArgs ea, float conf, hipStream_t s);
+hipError_t enqueue_learned_evalue(gx_ctx *c, gx_dqueries *q, const EvalArgs &ea, float conf, hipStream_t s);
 
 template <class T>
 int dalloc(T **p, size_t n) {

@@ -216,148 +216,217 @@ __device__ __forceinline__ double evalue_of(int32_t mu, int32_t beta, int32_t hy
     return e;
 }
 
+// The per-spectrum result before its e-value: the top hit unpacked from its
+// packed key, its tie flag, candidates, and whether the tail fit runs
+// (cpsms >= need, dslim_query.cpp:1017-1021).
+__device__ inline gx_psm top_hit_psm(unsigned long long key, uint32_t eq, uint32_t shp, uint32_t cps,
+                                     const DevBucket *buckets, uint32_t need) {
+    gx_psm r;
+    r.hyperscore = 0.0f;
+    r.psid = -1;
+    r.idxoffset = 0;
+    r.sharedions = 0;
+    r.totalions = 0;
+    r.flags = 0;
+    r.cpsms = (int32_t)cps;
+    r.hyp = 0;
+    r.mu = 0;
+    r.beta = 0;
+    r.evalue = INFINITY;
+    if (cps >= 1 && key) {
+        const uint32_t low = ~(uint32_t)(key & 0xffffffffu);
+        const uint32_t b = low >> 26, psid = low & ((1u << 26) - 1);
+        r.flags |= GX_PSM_HAS_HIT;
+        r.hyperscore = __uint_as_float((uint32_t)(key >> 32));
+        r.psid = (int32_t)psid;
+        r.idxoffset = (uint16_t)b;
+        r.sharedions = (uint16_t)shp;
+        r.totalions = (uint16_t)buckets[b].speclen;
+        r.hyp = hyp_bin(r.hyperscore);
+        if (eq > 1) r.flags |= GX_PSM_TIE;
+    }
+    if (cps >= need && r.psid >= 0) r.flags |= GX_PSM_MODELED;
+    return r;
+}
+
 struct FitScratch {
     int32_t i[8];
     double d[4];
 };
 
-// expeRT::ModelTailFit (expeRT.cpp:396-559) on a histogram in LDS. Called by
-// every thread of the block; wave 0 does the work, separated by block
-// barriers. The cumulative sum is over integer counts, so its parallel order
-// is exact; the regression sums run in lane 0 in the reference's order.
-// `sx` is LDS scratch for GX_HIST_BINS doubles. Returns fit_ok.
-__device__ bool tail_fit_block(const uint32_t *hist, double *sx, FitScratch &F, int hyp, int cpsms, double &slope,
-                               double &icept, int32_t &mu, int32_t &beta) {
-    const int tid = threadIdx.x, lane = tid & 63;
-    const bool w0 = tid < 64;
+// Orders one wave's LDS writes before its later LDS reads by other lanes of
+// the same wave (the LDS serves a wave's operations in order; the fences keep
+// the compiler from moving memory operations across this point).
+__device__ __forceinline__ void wave_lds_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
+// expeRT::ModelTailFit (expeRT.cpp:396-559) by one whole wave, on a histogram
+// in LDS or global memory. The cumulative sum is over integer counts, so its
+// parallel order is exact; the regression sums run in the reference's order
+// (every lane runs the same sequential sums). `sx` is LDS scratch for
+// GX_HIST_BINS doubles. Returns fit_ok; results are the same in every lane.
+__device__ bool tail_fit_wave(const uint32_t *hist, double *sx, int hyp, int cpsms, double &slope, double &icept,
+                              int32_t &mu, int32_t &beta) {
+    const int lane = threadIdx.x & 63;
     slope = icept = 0;
     mu = beta = 0;
     if (hyp < 0 || hyp >= GX_HIST_BINS || cpsms < 1) return false;
-    if (w0) {
-        int e = -1;
-        for (int p = lane; p <= hyp - 1; p += 64)
-            if (hist[p] >= 1) e = max(e, p);
-        e = wave_max_i32(e);
-        const int end1 = (e >= 0) ? e : hyp - 1;
-        int s = 0x7fffffff;
-        for (int p = lane; p <= end1; p += 64)
-            if (hist[p] >= 1) s = min(s, p);
-        s = wave_min_i32(s);
-        const int stt1 = (s != 0x7fffffff) ? s : 0;
-        if (lane == 0) {
-            F.i[0] = stt1;
-            F.i[1] = (stt1 == end1) ? end1 + 1 : end1;
-        }
-    }
-    __syncthreads();
-    const int stt1 = F.i[0], end1 = F.i[1];
+    int e = -1;
+    for (int p = lane; p <= hyp - 1; p += 64)
+        if (hist[p] >= 1) e = max(e, p);
+    e = wave_max_i32(e);
+    const int end0 = (e >= 0) ? e : hyp - 1;
+    int s = 0x7fffffff;
+    for (int p = lane; p <= end0; p += 64)
+        if (hist[p] >= 1) s = min(s, p);
+    s = wave_min_i32(s);
+    const int stt1 = (s != 0x7fffffff) ? s : 0;
+    const int end1 = (stt1 == end0) ? end0 + 1 : end0;
     const int n = end1 - stt1 + 1;
     if (n < 1) return false;
     const double vaa = (double)cpsms;
-    if (w0) {
-        uint32_t carry = 0;
-        for (int base = 0; base < n; base += 64) {
-            const int i = base + lane;
-            uint32_t v = (i < n && stt1 + i < GX_HIST_BINS) ? hist[stt1 + i] : 0u;
-            uint32_t inc = wave_incl_scan(v, lane) + carry;
-            if (i < n) {
-                double x = (double)inc;
-                x /= vaa;
-                x += -1.0;
-                x = -x;
-                if (x > 1.0) x = 0.999;
-                sx[i] = x;
-            }
-            carry = __shfl(inc, 63, 64);
+    uint32_t carry = 0;
+    for (int base = 0; base < n; base += 64) {
+        const int i = base + lane;
+        uint32_t v = (i < n && stt1 + i < GX_HIST_BINS) ? hist[stt1 + i] : 0u;
+        uint32_t inc = wave_incl_scan(v, lane) + carry;
+        if (i < n) {
+            double x = (double)inc;
+            x /= vaa;
+            x += -1.0;
+            x = -x;
+            if (x > 1.0) x = 0.999;
+            sx[i] = x;
         }
+        carry = __shfl(inc, 63, 64);
     }
-    __syncthreads();
-    if (w0) {
-        int r = -1;
+    int r = -1;
+    for (int i = lane; i < n; i += 64)
+        if (sx[i] >= 1e-4) r = max(r, i);
+    r = wave_max_i32(r);
+    const int repl = (r >= 0) ? r : n - 1;
+    wave_lds_sync();
+    const double rv = sx[repl];
+    for (int i = lane; i < n; i += 64) {
+        double x = sx[i];
+        if (x <= 0.0) x = rv;
+        sx[i] = log10(x);
+    }
+    wave_lds_sync();
+    const double s0 = sx[0];
+    const double hgt = sx[n - 1] - s0;
+    int mark = 0, mark2 = 0;
+    if (n >= 3) {
+        const double t1 = s0 + hgt * 0.22;
+        int f = 0x7fffffff;
         for (int i = lane; i < n; i += 64)
-            if (sx[i] >= 1e-4) r = max(r, i);
-        r = wave_max_i32(r);
-        const int repl = (r >= 0) ? r : n - 1;
-        if (lane == 0) F.d[2] = sx[repl];
-    }
-    __syncthreads();
-    if (w0) {
-        const double rv = F.d[2];
-        for (int i = lane; i < n; i += 64) {
-            double x = sx[i];
-            if (x <= 0.0) x = rv;
-            sx[i] = log10(x);
-        }
-    }
-    __syncthreads();
-    if (w0) {
-        const double s0 = sx[0];
-        const double hgt = sx[n - 1] - s0;
-        int mark = 0, mark2 = 0;
-        if (n >= 3) {
-            const double t1 = s0 + hgt * 0.22;
-            int f = 0x7fffffff;
+            if (sx[i] <= t1) f = min(f, i);
+        f = wave_min_i32(f);
+        mark = ((f != 0x7fffffff) ? f : 0) - 1;
+        if (n > 3) {
+            const double t2 = s0 + hgt * 0.87;
+            int g = -1;
             for (int i = lane; i < n; i += 64)
-                if (sx[i] <= t1) f = min(f, i);
-            f = wave_min_i32(f);
-            mark = ((f != 0x7fffffff) ? f : 0) - 1;
-            if (n > 3) {
-                const double t2 = s0 + hgt * 0.87;
-                int g = -1;
-                for (int i = lane; i < n; i += 64)
-                    if (sx[i] >= t2) g = max(g, i);
-                g = wave_max_i32(g);
-                mark2 = (g >= 0) ? g : n - 1;
-                if (mark2 == n) mark2 -= 1;
-            } else {
-                mark2 = n - 1;
-            }
-            if (mark >= mark2) mark = mark2 - 1;
+                if (sx[i] >= t2) g = max(g, i);
+            g = wave_max_i32(g);
+            mark2 = (g >= 0) ? g : n - 1;
+            if (mark2 == n) mark2 -= 1;
         } else {
-            mark = 0;
             mark2 = n - 1;
         }
-        if (lane == 0) {
-            int ok = 0;
-            double a = 0, b = 0;
-            if (mark >= 0 && mark2 >= mark) {
-                const int m = mark2 - mark + 1;
-                if (m == 1) {
-                    a = 0.0;
-                    b = sx[mark];
-                } else {
-                    double xbar = 0.0, ybar = 0.0;
-                    for (int i = 0; i < m; i++) {
-                        xbar = xbar + (double)(stt1 + mark + i);
-                        ybar = ybar + sx[mark + i];
-                    }
-                    xbar = xbar / (double)m;
-                    ybar = ybar / (double)m;
-                    double top = 0.0, bot = 0.0;
-                    for (int i = 0; i < m; i++) {
-                        const double xi = (double)(stt1 + mark + i);
-                        top = top + (xi - xbar) * (sx[mark + i] - ybar);
-                        bot = bot + (xi - xbar) * (xi - xbar);
-                    }
-                    a = top / bot;
-                    b = ybar - a * xbar;
-                }
-                const double am = a * 1e6, bm = b * 1e6;
-                ok = (fabs(am) < 2147483647.0 && fabs(bm) < 2147483647.0) ? 1 : 0;
-                F.i[5] = ok ? (int32_t)am : 0;
-                F.i[6] = ok ? (int32_t)bm : 0;
+        if (mark >= mark2) mark = mark2 - 1;
+    } else {
+        mark = 0;
+        mark2 = n - 1;
+    }
+    // Burkardt's llsq (expeRT.cpp:1241-1289) in the reference's sequential
+    // order. Every lane runs the same sums; the curve values are read 8 at a
+    // time (each read is an LDS broadcast: all lanes, one address) ahead of
+    // their 8 dependent additions, so the sums wait on the additions, not on
+    // a read per step.
+    if (!(mark >= 0 && mark2 >= mark)) return false;
+    // the bounds are the same in every lane; as scalars the loops below run
+    // on scalar control flow
+    mark = __builtin_amdgcn_readfirstlane(mark);
+    mark2 = __builtin_amdgcn_readfirstlane(mark2);
+    const int x0 = __builtin_amdgcn_readfirstlane(stt1) + mark;
+    const int m = mark2 - mark + 1;
+    double a = 0, b = 0;
+    if (m == 1) {
+        a = 0.0;
+        b = sx[mark];
+    } else {
+        double xbar = 0.0, ybar = 0.0;
+        int i = 0;
+        for (; i + 8 <= m; i += 8) {
+            double y[8];
+#pragma unroll
+            for (int u = 0; u < 8; u++) y[u] = sx[mark + i + u];
+#pragma unroll
+            for (int u = 0; u < 8; u++) {
+                xbar = xbar + (double)(x0 + i + u);
+                ybar = ybar + y[u];
             }
-            F.i[4] = ok;
-            F.d[0] = a;
-            F.d[1] = b;
+        }
+        for (; i < m; i++) {
+            xbar = xbar + (double)(x0 + i);
+            ybar = ybar + sx[mark + i];
+        }
+        xbar = xbar / (double)m;
+        ybar = ybar / (double)m;
+        double top = 0.0, bot = 0.0;
+        i = 0;
+        for (; i + 8 <= m; i += 8) {
+            double y[8];
+#pragma unroll
+            for (int u = 0; u < 8; u++) y[u] = sx[mark + i + u];
+#pragma unroll
+            for (int u = 0; u < 8; u++) {
+                const double xi = (double)(x0 + i + u);
+                top = top + (xi - xbar) * (y[u] - ybar);
+                bot = bot + (xi - xbar) * (xi - xbar);
+            }
+        }
+        for (; i < m; i++) {
+            const double xi = (double)(x0 + i);
+            top = top + (xi - xbar) * (sx[mark + i] - ybar);
+            bot = bot + (xi - xbar) * (xi - xbar);
+        }
+        a = top / bot;
+        b = ybar - a * xbar;
+    }
+    slope = a;
+    icept = b;
+    const double am = a * 1e6, bm = b * 1e6;
+    if (!(fabs(am) < 2147483647.0 && fabs(bm) < 2147483647.0)) return false;
+    mu = (int32_t)am;
+    beta = (int32_t)bm;
+    return true;
+}
+
+// The same for every thread of a block: wave 0 fits, the others wait and
+// read the result.
+__device__ bool tail_fit_block(const uint32_t *hist, double *sx, FitScratch &F, int hyp, int cpsms, double &slope,
+                               double &icept, int32_t &mu, int32_t &beta) {
+    if (threadIdx.x < 64) {
+        const bool ok = tail_fit_wave(hist, sx, hyp, cpsms, slope, icept, mu, beta);
+        if (threadIdx.x == 0) {
+            F.i[4] = ok ? 1 : 0;
+            F.i[5] = mu;
+            F.i[6] = beta;
+            F.d[0] = slope;
+            F.d[1] = icept;
         }
     }
     __syncthreads();
     const bool ok = F.i[4] != 0;
     slope = F.d[0];
     icept = F.d[1];
-    mu = ok ? F.i[5] : 0;
-    beta = ok ? F.i[6] : 0;
+    mu = F.i[5];
+    beta = F.i[6];
     __syncthreads();
     return ok;
 }

@@ -1,5 +1,16 @@
 // Learned e-value surrogate: a 128-256-256-3 GELU MLP on BF16 MFMA, trained
-// on the GPU from (survival-histogram features -> plain tail-fit) pairs.
+// on the GPU from (survival-curve features -> plain tail-fit window) pairs.
+//
+// The plain e-value model (expeRT::ModelTailFit, expeRT.cpp:396-559) is a
+// least-squares line through the log10 survival curve between two marks: one
+// before the first point at or below 22 % of the curve's drop (`mark`) and
+// the last point at or above 87 % of it (`mark2`), expeRT.cpp:477-515. The
+// network predicts where those
+// two marks lie (as fractions of the curve) and how sure it is. The learned
+// e-value stage checks each predicted mark against its defining inequality
+// in one wave-wide pass over the points around it, fits the line on the
+// window in parallel, and falls back to the plain fit on low confidence, a
+// short or degenerate curve, or a failed check.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -9,11 +20,16 @@
 
 namespace gx {
 
+struct DevBucket;
+
 constexpr int kFeat = 128;     // input features
 constexpr int kHidden = 256;   // hidden width
 constexpr int kOutPad = 16;    // 3 outputs padded to one MFMA tile
-constexpr int kOut = 3;        // slope, fitted log-survival at the top bin minus the anchor, confidence logit
-constexpr int kCurvePts = 120; // resampled points of the fit's log-survival curve
+constexpr int kOut = 3;        // mark position, mark2 position (fractions of the curve), confidence logit
+constexpr int kCurvePts = 120; // resampled points of the log-survival curve
+constexpr int kTarget = 4;     // per training row: mark, mark2, n - 1, servable
+constexpr int kMinCurve = 8;   // shortest curve the surrogate serves
+constexpr int kSnapLanes = 32; // points checked around each predicted mark
 
 typedef __bf16 bf16;
 
@@ -28,7 +44,17 @@ struct MlpLayout {
     static constexpr int N = B3 + kOutPad;
 };
 
-// Output normalisation: target = out * scale + shift (slope, u).
+// Weights as MFMA B fragments (bf16): for layer L, output tile nt and k step
+// ks, lane l holds W[ks*32 + 8*(l/16) + j][nt*16 + l%16], j < 8, in 16
+// contiguous bytes, so a wave loads one fragment with one coalesced load.
+struct FragLayout {
+    static constexpr int L1 = 0;                                          // 16 nt x 4 ks
+    static constexpr int L2 = L1 + (kHidden / 16) * (kFeat / 32) * 512;   // 16 nt x 8 ks
+    static constexpr int L3 = L2 + (kHidden / 16) * (kHidden / 32) * 512; // 1 nt x 8 ks
+    static constexpr int N = L3 + (kHidden / 32) * 512;
+};
+
+// Output normalisation: position = out * scale + shift (mark, mark2).
 struct MlpNorm {
     float a_shift, a_scale, u_shift, u_scale;
 };
@@ -52,21 +78,20 @@ struct GemmArgs {
 
 hipError_t launch_gemm(const GemmArgs &g, hipStream_t s);
 
-// features of searched spectra: the plain fit's log-survival curve resampled
-// to kCurvePts points, then stt1, end1, hyp, hyp - end1, curve length,
-// log10 cpsms, the anchor and the first curve value; targets from the plain fit
+// Training rows of searched spectra: the features, and as targets the plain
+// fit's exact marks, the curve length and whether the surrogate could serve
+// the spectrum at all (a curve of at least kMinCurve points with no
+// replaced values and marks that define a window).
 struct FeatArgs {
     uint32_t n;
     const uint32_t *hist;   // n * GX_HIST_BINS
-    const gx_psm *psm;      // top hits (and, for training, plain-fit mu, beta, flags)
+    const gx_psm *psm;      // top hits and the plain fit's mu, beta, flags
     bf16 *feat;             // n * kFeat
-    float *anchor;          // n: log10 survival at the curve's last bin (end1)
-    float *target;          // n * 2: slope, u - anchor with u = slope*hyp + intercept (truncated mu, beta)
-    float *loge;            // n: log10 of the plain e-value
+    float *target;          // n * kTarget: mark, mark2, n - 1, servable
+    float *loge;            // n: log10 of the plain e-value (0 if undefined)
     uint8_t *valid;         // n: a defined plain fit exists
-    uint8_t *sparse;        // n: not modeled, or fewer than kMinTailBins non-empty bins below the top bin
+    uint8_t *servable;      // n: target[3]
 };
-constexpr int kMinTailBins = 3;
 
 hipError_t launch_features(const FeatArgs &a, hipStream_t s);
 
@@ -74,13 +99,11 @@ hipError_t launch_features(const FeatArgs &a, hipStream_t s);
 struct LossArgs {
     uint32_t n;
     const float *out;       // n * kOutPad
-    const float *target;    // n * 2 (raw)
-    const float *loge;      // n
+    const float *target;    // n * kTarget
     MlpNorm norm;
     float conf_weight;
-    float tol;              // relative tolerance on log10 e for the confidence label
     bf16 *dout;             // n * kOutPad
-    float *loss;            // [4]: sum of squared errors (a, u), bce, count within tolerance
+    float *loss;            // [4]: squared errors (mark, mark2) over servable rows, bce, rows labelled 1
 };
 hipError_t launch_loss(const LossArgs &a, hipStream_t s);
 
@@ -88,24 +111,31 @@ hipError_t launch_colsum(const bf16 *x, uint32_t rows, uint32_t cols, float *out
 hipError_t launch_adam(float *w, const float *g, float *m, float *v, uint32_t n, float lr, float b1, float b2,
                        float eps, float b1t, float b2t, hipStream_t s);
 hipError_t launch_to_bf16(const float *x, bf16 *y, uint64_t n, hipStream_t s);
+hipError_t launch_frag_weights(const float *w, bf16 *frag, hipStream_t s);
 hipError_t launch_gather_rows(const bf16 *src, const float *tsrc, const float *lsrc, const uint32_t *idx, uint32_t n,
                               bf16 *dst, float *tdst, float *ldst, hipStream_t s);
 
-// route: accept the surrogate where confident, not sparse, in distribution
-// (normalised outputs within kMaxZ) and with a fitted log-survival u <= 0
-constexpr float kMaxZ = 4.0f;
-struct RouteArgs {
+// The learned e-value stage of a batched search, two kernels: per block of
+// 16 spectra the features and the three layers on MFMA (outputs to `raw`),
+// then per spectrum the window check and parallel fit, or the plain fit.
+struct LearnArgs {
     uint32_t n;
-    const float *out;       // n * kOutPad
-    const uint8_t *sparse;  // also set for spectra that are not modeled
-    const float *anchor;    // u = anchor + predicted residual
-    MlpNorm norm;
-    float conf_threshold;
+    const DevBucket *buckets;
+    int min_cpsm;
     double expect_max;
-    gx_psm *psm;            // updated in place for accepted spectra
-    uint8_t *route;         // 1 = surrogate accepted, 0 = plain fit
-    unsigned long long *stats;  // [ST_FALLBACK] counter row 0
+    const unsigned long long *best_key;
+    const uint32_t *best_eq;
+    const uint32_t *best_shp;
+    const uint32_t *cpsms;
+    const uint32_t *hist;
+    const bf16 *frag;       // FragLayout
+    const float *params;    // fp32 MlpLayout (biases)
+    MlpNorm norm;
+    float conf_logit;       // serve when the confidence logit is >= this
+    gx_psm *out;
+    float *raw;             // n * 4: outputs 0..2 and 1.0 where the surrogate served
+    unsigned long long *stats;  // ST_FALLBACK counter row 0
 };
-hipError_t launch_route(const RouteArgs &a, hipStream_t s);
+hipError_t launch_learned_evalue(const LearnArgs &a, hipStream_t s);
 
 }  // namespace gx

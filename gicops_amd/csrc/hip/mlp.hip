@@ -1,14 +1,12 @@
-// Learned e-value surrogate kernels for gfx950: survival-histogram features,
-// a strided BF16 GEMM on v_mfma_f32_16x16x32_bf16 with fused epilogues
-// (bias + GELU forward, GELU' backward), loss, bias gradients, Adam and the
-// confidence routing. The network predicts the reference tail fit's slope and
-// the fitted log-survival at the top hit's bin (so e = cpsms * 10^u), plus a
-// confidence logit; low confidence or a sparse histogram falls back to the
-// plain fit (expeRT::ModelTailFit, expeRT.cpp:396-559).
+// Learned e-value surrogate kernels for gfx950: a strided BF16 GEMM on
+// v_mfma_f32_16x16x32_bf16 with fused epilogues (bias + GELU forward, GELU'
+// backward) for training, the training rows of searched spectra (features
+// and the plain fit's exact marks, expeRT.cpp:477-515), the loss with the
+// window-check label, bias gradients, Adam, and the weights rearranged as
+// MFMA B fragments for the fused inference kernel (learned.hip).
 #include <math.h>
 
-#include "device_util.h"
-#include "mlp.h"
+#include "learned_util.h"
 
 namespace gx {
 namespace {
@@ -16,13 +14,9 @@ namespace {
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float floatx4 __attribute__((ext_vector_type(4)));
 
-__device__ __forceinline__ float gelu(float z) { return 0.5f * z * (1.0f + erff(z * 0.70710678118654752f)); }
+__device__ __forceinline__ float gelu(float z) { return mlp_gelu(z); }
 
-__device__ __forceinline__ float gelu_grad(float z) {
-    const float cdf = 0.5f * (1.0f + erff(z * 0.70710678118654752f));
-    const float pdf = 0.39894228040143268f * expf(-0.5f * z * z);
-    return cdf + z * pdf;
-}
+__device__ __forceinline__ float gelu_grad(float z) { return mlp_gelu_grad(z); }
 
 // One wave per 16 x (16*NT) output strip; K in steps of 32. Lane l supplies
 // A[row l%16][k0 + 8*(l/16) .. +7] and B[k0 + 8*(l/16) .. +7][col l%16] and
@@ -98,13 +92,13 @@ __global__ void __launch_bounds__(256) gemm_kernel(GemmArgs g) {
     }
 }
 
-// One wave per spectrum. The curve is the plain fit's own (expeRT.cpp:414-475):
-// bins from the first non-empty one (stt1) to the last non-empty one below
-// the top hit (end1), survival s = 1 - cumsum/cpsms from stt1, s > 1 -> 0.999,
-// s <= 0 replaced by the last value >= 1e-4, then log10. It is resampled to
-// kCurvePts points by linear interpolation; 8 scalars follow.
+// One wave per searched spectrum: the features (the inference kernel's own
+// function), the plain fit's exact marks over the whole curve in the plain
+// fit's arithmetic (expeRT.cpp:477-515), and the targets {mark before the
+// mark2 rule, mark2, n - 1, servable}.
 __global__ void __launch_bounds__(64) features_kernel(FeatArgs a) {
-    __shared__ double sx[GX_HIST_BINS];
+    __shared__ uint32_t cpre[GX_HIST_BINS];
+    __shared__ double lsx[GX_HIST_BINS];
     const int lane = threadIdx.x;
     const uint32_t s = blockIdx.x;
     if (s >= a.n) return;
@@ -114,86 +108,51 @@ __global__ void __launch_bounds__(64) features_kernel(FeatArgs a) {
     const uint32_t cps = (uint32_t)max(r.cpsms, 0);
     bf16 *f = a.feat + (size_t)s * kFeat;
     const bool modeled = (r.flags & GX_PSM_MODELED) && cps > 0 && hyp > 0 && hyp < GX_HIST_BINS;
-    int e = -1, st = 0x7fffffff;
-    uint32_t nonempty = 0;
+    Curve c{0, 0, 0, 0};
+    int serv = 0, mark = 0, mark2 = 0;
     if (modeled) {
-        for (int p = lane; p < hyp; p += 64) {
-            if (h[p] >= 1) {
-                e = max(e, p);
-                st = min(st, p);
-                nonempty++;
+        c = curve_bounds(h, hyp, lane);
+        curve_prefix(h, c, lane, cpre);
+        __syncthreads();
+        curve_features(cpre, c, hyp, cps, lane, f);
+        if (curve_servable(c, cpre, cps)) {
+            const double vaa = (double)cps;
+            for (int i = lane; i < c.n; i += 64) lsx[i] = log_survival(cpre[i], vaa);
+            __syncthreads();
+            const double s0 = lsx[0], hgt = lsx[c.n - 1] - s0;
+            const double t1 = s0 + hgt * 0.22, t2 = s0 + hgt * 0.87;
+            int fm = 0x7fffffff, g = -1;
+            for (int i = lane; i < c.n; i += 64) {
+                if (lsx[i] <= t1) fm = min(fm, i);
+                if (lsx[i] >= t2) g = max(g, i);
             }
+            fm = wave_min_i32(fm);
+            g = wave_max_i32(g);
+            mark = (fm != 0x7fffffff ? fm : 0) - 1;
+            mark2 = g >= 0 ? g : c.n - 1;
+            const int ruled = mark >= mark2 ? mark2 - 1 : mark;
+            serv = (fm != 0x7fffffff && fm >= 1 && ruled >= 0) ? 1 : 0;
         }
-    }
-    e = wave_max_i32(e);
-    st = wave_min_i32(st);
-    nonempty = wave_sum_u32(nonempty);
-    int end1 = e >= 0 ? e : hyp - 1;
-    const int stt1 = st != 0x7fffffff ? st : 0;
-    if (stt1 == end1) end1 += 1;
-    const int n = modeled ? end1 - stt1 + 1 : 0;
-    // survival from stt1
-    uint32_t carry = 0;
-    for (int b0 = 0; b0 < n; b0 += 64) {
-        const int i = b0 + lane;
-        const uint32_t v = (i < n && stt1 + i < GX_HIST_BINS) ? h[stt1 + i] : 0u;
-        const uint32_t inc = wave_incl_scan(v, lane) + carry;
-        if (i < n) {
-            double x = -((double)inc / (double)cps - 1.0);
-            sx[i] = x > 1.0 ? 0.999 : x;
-        }
-        carry = __shfl(inc, 63, 64);
-    }
-    __syncthreads();
-    int rp = -1;
-    for (int i = lane; i < n; i += 64)
-        if (sx[i] >= 1e-4) rp = max(rp, i);
-    rp = wave_max_i32(rp);
-    const double rv = n > 0 ? sx[rp >= 0 ? rp : n - 1] : 1.0;
-    __syncthreads();
-    int bad = 0;
-    for (int i = lane; i < n; i += 64) {
-        const double x = sx[i] <= 0.0 ? rv : sx[i];
-        const double l = log10(x);
-        bad |= isfinite(l) ? 0 : 1;
-        sx[i] = isfinite(l) ? l : -8.0;
-    }
-    bad = wave_max_i32(bad);
-    __syncthreads();
-    for (int j = lane; j < kCurvePts; j += 64) {
-        float v = 0.0f;
-        if (n > 0) {
-            const double t = n > 1 ? (double)j * (double)(n - 1) / (double)(kCurvePts - 1) : 0.0;
-            const int i0 = min((int)t, n - 1);
-            const int i1 = min(i0 + 1, n - 1);
-            const double fr = t - (double)i0;
-            v = (float)((sx[i0] * (1.0 - fr) + sx[i1] * fr) / 4.0);
-        }
-        f[j] = (bf16)v;
+    } else {
+        for (int j = lane; j < kFeat; j += 64) f[j] = (bf16)0.0f;
     }
     if (lane == 0) {
-        const double anchor = n > 0 ? sx[n - 1] : 0.0;
-        const double lc = cps > 0 ? log10((double)cps) : 0.0;
-        f[kCurvePts + 0] = (bf16)(n > 0 ? (float)stt1 / 800.0f : 0.0f);
-        f[kCurvePts + 1] = (bf16)(n > 0 ? (float)end1 / 800.0f : 0.0f);
-        f[kCurvePts + 2] = (bf16)(n > 0 ? (float)hyp / 800.0f : 0.0f);
-        f[kCurvePts + 3] = (bf16)(n > 0 ? (float)(hyp - end1) / 400.0f : 0.0f);
-        f[kCurvePts + 4] = (bf16)(n > 0 ? (float)n / 400.0f : 0.0f);
-        f[kCurvePts + 5] = (bf16)(n > 0 ? (float)(lc / 4.0) : 0.0f);
-        f[kCurvePts + 6] = (bf16)(n > 0 ? (float)(anchor / 4.0) : 0.0f);
-        f[kCurvePts + 7] = (bf16)(n > 0 ? (float)sx[0] : 0.0f);
         const bool fit = (r.flags & GX_PSM_FIT_OK) != 0;
-        const double slope = (double)r.mu / 1e6;
-        const double u = slope * hyp + (double)r.beta / 1e6;
-        a.anchor[s] = (float)anchor;
-        a.target[2 * (size_t)s] = (float)slope;
-        a.target[2 * (size_t)s + 1] = (float)(u - anchor);
-        a.loge[s] = fit ? (float)(lc + u) : 0.0f;
+        const double u = (double)r.mu / 1e6 * hyp + (double)r.beta / 1e6;
+        float *t = a.target + (size_t)s * kTarget;
+        t[0] = (float)mark;
+        t[1] = (float)mark2;
+        t[2] = (float)max(c.n - 1, 0);
+        t[3] = (float)serv;
+        a.loge[s] = (fit && cps > 0) ? (float)(log10((double)cps) + u) : 0.0f;
         a.valid[s] = (modeled && fit) ? 1 : 0;
-        a.sparse[s] = (!modeled || bad || nonempty < (uint32_t)kMinTailBins) ? 1 : 0;
+        a.servable[s] = (uint8_t)serv;
     }
 }
 
+// Mean squared error of the two normalised positions (servable rows) plus
+// conf_weight times the cross-entropy of the confidence against the label
+// "both marks lie in the check windows around the predicted indexes".
 __global__ void loss_kernel(LossArgs a) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     __shared__ float acc[4];
@@ -201,22 +160,28 @@ __global__ void loss_kernel(LossArgs a) {
     __syncthreads();
     if (i < a.n) {
         const float *o = a.out + (size_t)i * kOutPad;
-        const float ta = (a.target[2 * (size_t)i] - a.norm.a_shift) / a.norm.a_scale;
-        const float tu = (a.target[2 * (size_t)i + 1] - a.norm.u_shift) / a.norm.u_scale;
-        const float ea = o[0] - ta, eu = o[1] - tu;
-        const float err = fabsf(eu) * a.norm.u_scale;  // |u_pred - u| = error of log10 e
-        const float tol = a.tol * fmaxf(fabsf(a.loge[i]), 1e-3f);
-        const float y = err <= tol ? 1.0f : 0.0f;
+        const float *t = a.target + (size_t)i * kTarget;
+        const bool serv = t[3] > 0.5f;
+        float e0 = 0.0f, e1 = 0.0f, y = 0.0f;
+        if (serv) {
+            const float nm1 = t[2];
+            const int n = (int)nm1 + 1;
+            e0 = o[0] - (t[0] / nm1 - a.norm.a_shift) / a.norm.a_scale;
+            e1 = o[1] - (t[1] / nm1 - a.norm.u_shift) / a.norm.u_scale;
+            const int q1 = predicted_index(o[0], a.norm.a_scale, a.norm.a_shift, n);
+            const int q2 = predicted_index(o[1], a.norm.u_scale, a.norm.u_shift, n);
+            y = snap_label(q1, q2, (int)t[0], (int)t[1], n) ? 1.0f : 0.0f;
+        }
         const float p = 1.0f / (1.0f + expf(-o[2]));
         const float bce = -(y * logf(fmaxf(p, 1e-7f)) + (1.0f - y) * logf(fmaxf(1.0f - p, 1e-7f)));
         const float inv = 1.0f / (float)a.n;
         bf16 *d = a.dout + (size_t)i * kOutPad;
-        d[0] = (bf16)(2.0f * ea * inv);
-        d[1] = (bf16)(2.0f * eu * inv);
+        d[0] = (bf16)(2.0f * e0 * inv);
+        d[1] = (bf16)(2.0f * e1 * inv);
         d[2] = (bf16)(a.conf_weight * (p - y) * inv);
         for (int j = 3; j < kOutPad; j++) d[j] = (bf16)0.0f;
-        atomicAdd(&acc[0], ea * ea);
-        atomicAdd(&acc[1], eu * eu);
+        atomicAdd(&acc[0], e0 * e0);
+        atomicAdd(&acc[1], e1 * e1);
         atomicAdd(&acc[2], bce);
         atomicAdd(&acc[3], y);
     }
@@ -224,12 +189,24 @@ __global__ void loss_kernel(LossArgs a) {
     if (threadIdx.x < 4) atomicAdd(&a.loss[threadIdx.x], acc[threadIdx.x]);
 }
 
-__global__ void colsum_kernel(const bf16 *x, uint32_t rows, uint32_t cols, float *out) {
-    const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= cols) return;
+// column sums of a bf16 [rows][cols] matrix (bias gradients): one block per
+// 64 columns, its 16 waves over interleaved rows (a wave reads 128 B of one
+// row), then a reduction over the waves in LDS
+__global__ void __launch_bounds__(1024) colsum_kernel(const bf16 *x, uint32_t rows, uint32_t cols, float *out) {
+    __shared__ float part[16][64];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const uint32_t c = blockIdx.x * 64 + lane;
     float s = 0.0f;
-    for (uint32_t r = 0; r < rows; r++) s += (float)x[(size_t)r * cols + c];
-    out[c] = s;
+    if (c < cols)
+        for (uint32_t r = w; r < rows; r += 16) s += (float)x[(size_t)r * cols + c];
+    part[w][lane] = s;
+    __syncthreads();
+    if (w == 0 && c < cols) {
+        float t = 0.0f;
+#pragma unroll
+        for (int k = 0; k < 16; k++) t += part[k][lane];
+        out[c] = t;
+    }
 }
 
 __global__ void adam_kernel(float *w, const float *g, float *m, float *v, uint32_t n, float lr, float b1, float b2,
@@ -256,48 +233,28 @@ __global__ void gather_rows_kernel(const bf16 *src, const float *tsrc, const flo
     if (i >= n) return;
     const uint32_t j = idx[i];
     for (int k = threadIdx.x; k < kFeat; k += blockDim.x) dst[(size_t)i * kFeat + k] = src[(size_t)j * kFeat + k];
-    if (threadIdx.x == 0) {
-        tdst[2 * (size_t)i] = tsrc[2 * (size_t)j];
-        tdst[2 * (size_t)i + 1] = tsrc[2 * (size_t)j + 1];
-        ldst[i] = lsrc[j];
-    }
+    if (threadIdx.x < kTarget) tdst[(size_t)i * kTarget + threadIdx.x] = tsrc[(size_t)j * kTarget + threadIdx.x];
+    if (threadIdx.x == 0) ldst[i] = lsrc[j];
 }
 
-__global__ void route_kernel(RouteArgs a) {
-    const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
-    unsigned long long fb = 0;
-    if (s < a.n) {
-        uint8_t accept = 0;
-        const bool modeled = (a.psm[s].flags & GX_PSM_MODELED) != 0;
-        if (modeled && !a.sparse[s]) {
-            const float *o = a.out + (size_t)s * kOutPad;
-            const float p = 1.0f / (1.0f + expf(-o[2]));
-            // outputs beyond kMaxZ standard deviations of the training targets
-            // are out of distribution, whatever the confidence says
-            const bool in_range = fabsf(o[0]) <= kMaxZ && fabsf(o[1]) <= kMaxZ;
-            if (p >= a.conf_threshold && in_range) {
-                gx_psm r = a.psm[s];
-                const double slope = (double)o[0] * a.norm.a_scale + a.norm.a_shift;
-                const double u = (double)a.anchor[s] + (double)o[1] * a.norm.u_scale + a.norm.u_shift;
-                const double e = (double)r.cpsms * pow(10.0, u);
-                const double am = slope * 1e6, bm = (u - slope * r.hyp) * 1e6;
-                // a survival probability: u <= 0
-                if (u <= 0.0 && isfinite(e) && fabs(am) < 2147483647.0 && fabs(bm) < 2147483647.0) {
-                    r.mu = (int32_t)am;
-                    r.beta = (int32_t)bm;
-                    r.evalue = e;
-                    r.flags |= GX_PSM_FIT_OK | GX_PSM_SURROGATE;
-                    if (e < a.expect_max) r.flags |= GX_PSM_REPORTED;
-                    a.psm[s] = r;
-                    accept = 1;
-                }
-            }
-        }
-        a.route[s] = accept;
-        fb = (modeled && !accept) ? 1ull : 0ull;
+// the weights as MFMA B fragments (FragLayout), rounded to bf16 as the
+// training GEMMs round them
+__global__ void frag_weights_kernel(const float *w, bf16 *frag) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= FragLayout::N) return;
+    int base, ksteps, ncols, src;
+    if (i < FragLayout::L2) {
+        base = FragLayout::L1, ksteps = kFeat / 32, ncols = kHidden, src = MlpLayout::W1;
+    } else if (i < FragLayout::L3) {
+        base = FragLayout::L2, ksteps = kHidden / 32, ncols = kHidden, src = MlpLayout::W2;
+    } else {
+        base = FragLayout::L3, ksteps = kHidden / 32, ncols = kOutPad, src = MlpLayout::W3;
     }
-    fb = wave_sum_u64(fb);
-    if ((threadIdx.x & 63) == 0 && fb) atomicAdd(&a.stats[ST_FALLBACK], fb);
+    const int local = i - base, fid = local / 512, within = local % 512;
+    const int ln = within / 8, j = within % 8;
+    const int nt = fid / ksteps, ks = fid % ksteps;
+    const int k = ks * 32 + 8 * (ln / 16) + j, col = nt * 16 + ln % 16;
+    frag[i] = (bf16)w[src + k * ncols + col];
 }
 
 }  // namespace
@@ -333,7 +290,7 @@ hipError_t launch_loss(const LossArgs &a, hipStream_t s) {
 }
 
 hipError_t launch_colsum(const bf16 *x, uint32_t rows, uint32_t cols, float *out, hipStream_t s) {
-    hipLaunchKernelGGL(colsum_kernel, dim3((cols + 63) / 64), dim3(64), 0, s, x, rows, cols, out);
+    hipLaunchKernelGGL(colsum_kernel, dim3((cols + 63) / 64), dim3(1024), 0, s, x, rows, cols, out);
     return hipGetLastError();
 }
 
@@ -358,9 +315,8 @@ hipError_t launch_gather_rows(const bf16 *src, const float *tsrc, const float *l
     return hipGetLastError();
 }
 
-hipError_t launch_route(const RouteArgs &a, hipStream_t s) {
-    if (a.n == 0) return hipSuccess;
-    hipLaunchKernelGGL(route_kernel, dim3((a.n + 255) / 256), dim3(256), 0, s, a);
+hipError_t launch_frag_weights(const float *w, bf16 *frag, hipStream_t s) {
+    hipLaunchKernelGGL(frag_weights_kernel, dim3((FragLayout::N + 255) / 256), dim3(256), 0, s, w, frag);
     return hipGetLastError();
 }
 

@@ -1,7 +1,7 @@
 // C ABI of the learned e-value surrogate (include/gicops_amd.h, gx_mlp_*):
 // parameters, training data taken from searched batches, training on the GPU
 // with the project's MFMA forward/backward kernels, one-shot gradients for
-// the CPU check, and inference inside the batched search.
+// the CPU check, and the fused inference stage inside the batched search.
 #include <algorithm>
 #include <cmath>
 #include <new>
@@ -17,7 +17,8 @@ namespace gx {
 struct Surrogate {
     gx_ctx *ctx = nullptr;
     float *w = nullptr, *g = nullptr, *m = nullptr, *v = nullptr;
-    bf16 *wb = nullptr;
+    bf16 *wb = nullptr;    // bf16 copy in MlpLayout (training GEMMs)
+    bf16 *frag = nullptr;  // bf16 MFMA B fragments (FragLayout, inference)
     MlpNorm norm{0.0f, 1.0f, 0.0f, 1.0f};
     bool norm_set = false;
     long long step = 0;
@@ -52,7 +53,7 @@ int ensure_work(gx_mlp *M, uint32_t n) {
     M->wcap = 0;
     if (dalloc(&M->x, (size_t)n * kFeat) || dalloc(&M->h1, (size_t)n * kHidden) || dalloc(&M->h2, (size_t)n * kHidden) ||
         dalloc(&M->dz1, (size_t)n * kHidden) || dalloc(&M->dz2, (size_t)n * kHidden) ||
-        dalloc(&M->dout, (size_t)n * kOutPad) || dalloc(&M->tg, (size_t)n * 2) || dalloc(&M->lg, n) ||
+        dalloc(&M->dout, (size_t)n * kOutPad) || dalloc(&M->tg, (size_t)n * kTarget) || dalloc(&M->lg, n) ||
         dalloc(&M->z1, (size_t)n * kHidden) || dalloc(&M->z2, (size_t)n * kHidden) ||
         dalloc(&M->out, (size_t)n * kOutPad) || dalloc(&M->idx, n))
         return GX_ERR_BAD_MEM_ALLOC;
@@ -189,8 +190,9 @@ extern "C" int gx_mlp_create(gx_ctx *c, gx_mlp **out) {
     if (!M) return GX_ERR_BAD_MEM_ALLOC;
     M->ctx = c;
     if (dalloc(&M->w, MlpLayout::N) || dalloc(&M->g, MlpLayout::N) || dalloc(&M->m, MlpLayout::N) ||
-        dalloc(&M->v, MlpLayout::N) || dalloc(&M->wb, MlpLayout::N) || dalloc(&M->loss, 4)) {
-        free_all({M->w, M->g, M->m, M->v, M->wb, M->loss});
+        dalloc(&M->v, MlpLayout::N) || dalloc(&M->wb, MlpLayout::N) || dalloc(&M->frag, FragLayout::N) ||
+        dalloc(&M->loss, 4)) {
+        free_all({M->w, M->g, M->m, M->v, M->wb, M->frag, M->loss});
         delete M;
         return GX_ERR_BAD_MEM_ALLOC;
     }
@@ -206,7 +208,7 @@ extern "C" void gx_mlp_destroy(gx_mlp *M) {
         M->ctx->surrogate = nullptr;
         M->ctx->scratch_gen++;
     }
-    free_all({M->w, M->g, M->m, M->v, M->wb, M->loss, M->ds_feat, M->ds_tgt, M->ds_loge, M->x, M->h1, M->h2, M->dz1,
+    free_all({M->w, M->g, M->m, M->v, M->wb, M->frag, M->loss, M->ds_feat, M->ds_tgt, M->ds_loge, M->x, M->h1, M->h2, M->dz1,
               M->dz2, M->dout, M->tg, M->lg, M->z1, M->z2, M->out, M->idx});
     delete M;
 }
@@ -219,6 +221,7 @@ static int upload_params(gx_mlp *M, const std::vector<float> &h) {
     GX_HIP(hipMemcpy(M->m, z.data(), sizeof(float) * MlpLayout::N, hipMemcpyHostToDevice));
     GX_HIP(hipMemcpy(M->v, z.data(), sizeof(float) * MlpLayout::N, hipMemcpyHostToDevice));
     GX_HIP(launch_to_bf16(M->w, M->wb, MlpLayout::N, s));
+    GX_HIP(launch_frag_weights(M->w, M->frag, s));
     GX_HIP(hipStreamSynchronize(s));
     M->step = 0;
     M->ctx->scratch_gen++;
@@ -275,8 +278,8 @@ namespace {
 int alloc_features(gx_dqueries *q) {
     if (q->l_feat) return GX_SUCCESS;
     const size_t ns = q->n_spectra;
-    if (dalloc(&q->l_feat, ns * kFeat) || dalloc(&q->l_anchor, ns) || dalloc(&q->l_tgt, ns * 2) ||
-        dalloc(&q->l_loge, ns) || dalloc(&q->l_valid, ns) || dalloc(&q->l_sparse, ns))
+    if (dalloc(&q->l_feat, ns * kFeat) || dalloc(&q->l_tgt, ns * kTarget) || dalloc(&q->l_loge, ns) ||
+        dalloc(&q->l_valid, ns) || dalloc(&q->l_serv, ns))
         return GX_ERR_BAD_MEM_ALLOC;
     q->buf_gen++;
     return GX_SUCCESS;
@@ -288,15 +291,14 @@ FeatArgs feat_args(gx_dqueries *q) {
     fa.hist = q->hist;
     fa.psm = q->psm;
     fa.feat = q->l_feat;
-    fa.anchor = q->l_anchor;
     fa.target = q->l_tgt;
     fa.loge = q->l_loge;
     fa.valid = q->l_valid;
-    fa.sparse = q->l_sparse;
+    fa.servable = q->l_serv;
     return fa;
 }
 
-// features of a searched batch (its plain-fit results are the targets)
+// features and plain-fit targets of a searched batch
 int batch_features(gx_ctx *c, gx_dqueries *q) {
     int st = alloc_features(q);
     if (st) return st;
@@ -314,24 +316,27 @@ extern "C" int gx_mlp_add_training(gx_mlp *M, gx_dqueries *q, uint64_t *n_added)
     if (hipSetDevice(c->device) != hipSuccess) return GX_ERR_DEVICE;
     int st = batch_features(c, q);
     if (st) return st;
-    std::vector<uint8_t> valid(q->n_spectra);
+    // the spectra the surrogate could serve: a servable curve and a defined plain fit
+    std::vector<uint8_t> valid(q->n_spectra), serv(q->n_spectra);
     GX_HIP(hipMemcpyAsync(valid.data(), q->l_valid, q->n_spectra, hipMemcpyDeviceToHost, c->stream));
+    GX_HIP(hipMemcpyAsync(serv.data(), q->l_serv, q->n_spectra, hipMemcpyDeviceToHost, c->stream));
     GX_HIP(hipStreamSynchronize(c->stream));
     std::vector<uint32_t> sel;
     for (uint32_t i = 0; i < q->n_spectra; i++)
-        if (valid[i]) sel.push_back(i);
+        if (valid[i] && serv[i]) sel.push_back(i);
     const uint32_t add = (uint32_t)sel.size();
     if (M->ds_n + add > M->ds_cap) {
         const uint32_t cap = std::max<uint32_t>(M->ds_n + add, M->ds_cap * 2 + 4096);
         bf16 *f = nullptr;
         float *t = nullptr, *l = nullptr;
-        if (dalloc(&f, (size_t)cap * kFeat) || dalloc(&t, (size_t)cap * 2) || dalloc(&l, cap)) {
+        if (dalloc(&f, (size_t)cap * kFeat) || dalloc(&t, (size_t)cap * kTarget) || dalloc(&l, cap)) {
             free_all({f, t, l});
             return GX_ERR_BAD_MEM_ALLOC;
         }
         if (M->ds_n) {
             GX_HIP(hipMemcpyAsync(f, M->ds_feat, sizeof(bf16) * M->ds_n * kFeat, hipMemcpyDeviceToDevice, c->stream));
-            GX_HIP(hipMemcpyAsync(t, M->ds_tgt, sizeof(float) * M->ds_n * 2, hipMemcpyDeviceToDevice, c->stream));
+            GX_HIP(hipMemcpyAsync(t, M->ds_tgt, sizeof(float) * M->ds_n * kTarget, hipMemcpyDeviceToDevice,
+                                  c->stream));
             GX_HIP(hipMemcpyAsync(l, M->ds_loge, sizeof(float) * M->ds_n, hipMemcpyDeviceToDevice, c->stream));
             GX_HIP(hipStreamSynchronize(c->stream));
         }
@@ -346,8 +351,9 @@ extern "C" int gx_mlp_add_training(gx_mlp *M, gx_dqueries *q, uint64_t *n_added)
         if (dalloc(&d_sel, add)) return GX_ERR_BAD_MEM_ALLOC;
         GX_HIP(hipMemcpy(d_sel, sel.data(), sizeof(uint32_t) * add, hipMemcpyHostToDevice));
         const hipError_t e = launch_gather_rows(q->l_feat, q->l_tgt, q->l_loge, d_sel, add,
-                                                M->ds_feat + (size_t)M->ds_n * kFeat, M->ds_tgt + (size_t)M->ds_n * 2,
-                                                M->ds_loge + M->ds_n, c->stream);
+                                                M->ds_feat + (size_t)M->ds_n * kFeat,
+                                                M->ds_tgt + (size_t)M->ds_n * kTarget, M->ds_loge + M->ds_n,
+                                                c->stream);
         const hipError_t e2 = hipStreamSynchronize(c->stream);
         (void)hipFree(d_sel);
         if (e != hipSuccess || e2 != hipSuccess) {
@@ -371,19 +377,24 @@ extern "C" int gx_mlp_train(gx_mlp *M, const gx_train_params *tp, gx_train_repor
     const uint32_t N = M->ds_n, B = (uint32_t)tp->batch;
     int st = ensure_work(M, B);
     if (st) return st;
-    if (!M->norm_set) {  // standardise the regression targets
-        std::vector<float> t((size_t)N * 2);
+    if (!M->norm_set) {  // standardise the two positions (fractions of the curve) over the training rows
+        std::vector<float> t((size_t)N * kTarget);
         GX_HIP(hipMemcpy(t.data(), M->ds_tgt, sizeof(float) * t.size(), hipMemcpyDeviceToHost));
-        double sa = 0, sa2 = 0, su = 0, su2 = 0;
+        double s1 = 0, s1q = 0, s2 = 0, s2q = 0, cnt = 0;
         for (uint32_t i = 0; i < N; i++) {
-            sa += t[2 * i];
-            sa2 += (double)t[2 * i] * t[2 * i];
-            su += t[2 * i + 1];
-            su2 += (double)t[2 * i + 1] * t[2 * i + 1];
+            const float *r = &t[(size_t)i * kTarget];
+            if (r[3] < 0.5f || r[2] <= 0.0f) continue;
+            const double p1 = r[0] / r[2], p2 = r[1] / r[2];
+            s1 += p1;
+            s1q += p1 * p1;
+            s2 += p2;
+            s2q += p2 * p2;
+            cnt += 1;
         }
-        const double ma = sa / N, mu = su / N;
-        M->norm = MlpNorm{(float)ma, (float)std::sqrt(std::max(sa2 / N - ma * ma, 1e-12)), (float)mu,
-                          (float)std::sqrt(std::max(su2 / N - mu * mu, 1e-12))};
+        cnt = std::max(cnt, 1.0);
+        const double m1 = s1 / cnt, m2 = s2 / cnt;
+        M->norm = MlpNorm{(float)m1, (float)std::sqrt(std::max(s1q / cnt - m1 * m1, 1e-8)), (float)m2,
+                          (float)std::sqrt(std::max(s2q / cnt - m2 * m2, 1e-8))};
         M->norm_set = true;
     }
     if (rep) std::memset(rep, 0, sizeof(*rep));
@@ -403,7 +414,7 @@ extern "C" int gx_mlp_train(gx_mlp *M, const gx_train_params *tp, gx_train_repor
             rc = GX_ERR_DEVICE;
             break;
         }
-        if (hipMemsetAsync(M->loss, 0, sizeof(float) * 4, s) != hipSuccess) {
+        if (launch_fill_u32(reinterpret_cast<uint32_t *>(M->loss), 4, 0, s) != hipSuccess) {
             rc = GX_ERR_DEVICE;
             break;
         }
@@ -411,7 +422,7 @@ extern "C" int gx_mlp_train(gx_mlp *M, const gx_train_params *tp, gx_train_repor
         for (uint32_t b0 = 0; b0 + B <= N; b0 += B) {
             hipError_t e = launch_gather_rows(M->ds_feat, M->ds_tgt, M->ds_loge, d_perm + b0, B, M->x, M->tg, M->lg, s);
             if (!e) e = forward(M, M->x, B, s, true, M->h1, M->h2, M->z1, M->z2, M->out);
-            LossArgs la{B, M->out, M->tg, M->lg, M->norm, tp->conf_weight, tp->tol, M->dout, M->loss};
+            LossArgs la{B, M->out, M->tg, M->norm, tp->conf_weight, M->dout, M->loss};
             if (!e) e = launch_loss(la, s);
             if (!e) e = backward(M, M->x, B, s);
             M->step++;
@@ -430,13 +441,14 @@ extern "C" int gx_mlp_train(gx_mlp *M, const gx_train_params *tp, gx_train_repor
             rc = GX_ERR_DEVICE;
         if (rc == GX_SUCCESS && rep) {
             const float nb = (float)((N / B) * B);
-            rep->mse_slope[ep] = l[0] / nb;
-            rep->mse_u[ep] = l[1] / nb;
+            rep->mse_mark[ep] = l[0] / nb;
+            rep->mse_mark2[ep] = l[1] / nb;
             rep->bce[ep] = l[2] / nb;
-            rep->within_tol[ep] = l[3] / nb;
+            rep->in_window[ep] = l[3] / nb;
             rep->epochs = ep + 1;
         }
     }
+    if (rc == GX_SUCCESS && launch_frag_weights(M->w, M->frag, s) != hipSuccess) rc = GX_ERR_DEVICE;
     (void)hipStreamSynchronize(s);
     (void)hipFree(d_perm);
     if (rep) rep->samples = N;
@@ -444,9 +456,9 @@ extern "C" int gx_mlp_train(gx_mlp *M, const gx_train_params *tp, gx_train_repor
     return rc;
 }
 
-extern "C" int gx_mlp_grad(gx_mlp *M, const float *feat, const float *target, const float *loge, uint32_t n,
-                           const float *norm, float conf_weight, float tol, float *grad, float *out, float *loss) {
-    if (!M || !feat || !target || !loge || !norm || !grad) return GX_ERR_INVLD_PTR;
+extern "C" int gx_mlp_grad(gx_mlp *M, const float *feat, const float *target, uint32_t n, const float *norm,
+                           float conf_weight, float *grad, float *out, float *loss) {
+    if (!M || !feat || !target || !norm || !grad) return GX_ERR_INVLD_PTR;
     if (n == 0) return GX_ERR_INVLD_SIZE;
     gx_ctx *c = M->ctx;
     std::lock_guard<std::mutex> lk(c->mu);
@@ -459,12 +471,11 @@ extern "C" int gx_mlp_grad(gx_mlp *M, const float *feat, const float *target, co
     const MlpNorm nm{norm[0], norm[1], norm[2], norm[3]};
     hipError_t e = hipStreamSynchronize(s);
     if (!e) e = hipMemcpy(fx, feat, sizeof(float) * n * kFeat, hipMemcpyHostToDevice);
-    if (!e) e = hipMemcpy(M->tg, target, sizeof(float) * n * 2, hipMemcpyHostToDevice);
-    if (!e) e = hipMemcpy(M->lg, loge, sizeof(float) * n, hipMemcpyHostToDevice);
+    if (!e) e = hipMemcpy(M->tg, target, sizeof(float) * n * kTarget, hipMemcpyHostToDevice);
     if (!e) e = launch_to_bf16(fx, M->x, (uint64_t)n * kFeat, s);
-    if (!e) e = hipMemsetAsync(M->loss, 0, sizeof(float) * 4, s);
+    if (!e) e = launch_fill_u32(reinterpret_cast<uint32_t *>(M->loss), 4, 0, s);
     if (!e) e = forward(M, M->x, n, s, true, M->h1, M->h2, M->z1, M->z2, M->out);
-    LossArgs la{n, M->out, M->tg, M->lg, nm, conf_weight, tol, M->dout, M->loss};
+    LossArgs la{n, M->out, M->tg, nm, conf_weight, M->dout, M->loss};
     if (!e) e = launch_loss(la, s);
     if (!e) e = backward(M, M->x, n, s);
     if (!e) e = hipStreamSynchronize(s);
@@ -513,8 +524,8 @@ extern "C" int gx_ctx_set_surrogate(gx_ctx *c, gx_mlp *M) {
     return GX_SUCCESS;
 }
 
-extern "C" int gx_features_download(gx_ctx *c, gx_dqueries *q, float *feat, float *anchor, float *target,
-                                    float *loge, uint8_t *valid, uint8_t *sparse) {
+extern "C" int gx_features_download(gx_ctx *c, gx_dqueries *q, float *feat, float *target, float *loge,
+                                    uint8_t *valid, uint8_t *servable) {
     if (!c || !q) return GX_ERR_INVLD_PTR;
     if (!q->searched || !q->hist) return GX_ERR_NOT_READY;
     std::lock_guard<std::mutex> lk(c->mu);
@@ -528,54 +539,55 @@ extern "C" int gx_features_download(gx_ctx *c, gx_dqueries *q, float *feat, floa
         GX_HIP(hipMemcpy(f.data(), q->l_feat, sizeof(bf16) * f.size(), hipMemcpyDeviceToHost));
         for (size_t i = 0; i < f.size(); i++) feat[i] = (float)f[i];
     }
-    if (anchor) GX_HIP(hipMemcpy(anchor, q->l_anchor, sizeof(float) * ns, hipMemcpyDeviceToHost));
-    if (target) GX_HIP(hipMemcpy(target, q->l_tgt, sizeof(float) * ns * 2, hipMemcpyDeviceToHost));
+    if (target) GX_HIP(hipMemcpy(target, q->l_tgt, sizeof(float) * ns * kTarget, hipMemcpyDeviceToHost));
     if (loge) GX_HIP(hipMemcpy(loge, q->l_loge, sizeof(float) * ns, hipMemcpyDeviceToHost));
     if (valid) GX_HIP(hipMemcpy(valid, q->l_valid, ns, hipMemcpyDeviceToHost));
-    if (sparse) GX_HIP(hipMemcpy(sparse, q->l_sparse, ns, hipMemcpyDeviceToHost));
+    if (servable) GX_HIP(hipMemcpy(servable, q->l_serv, ns, hipMemcpyDeviceToHost));
+    return GX_SUCCESS;
+}
+
+extern "C" int gx_surrogate_outputs_download(gx_ctx *c, gx_dqueries *q, float *out) {
+    if (!c || !q || !out) return GX_ERR_INVLD_PTR;
+    if (!q->searched || !q->l_out || !q->learned_done) return GX_ERR_NOT_READY;
+    std::lock_guard<std::mutex> lk(c->mu);
+    if (hipSetDevice(c->device) != hipSuccess) return GX_ERR_DEVICE;
+    GX_HIP(hipStreamSynchronize(c->stream));
+    GX_HIP(hipMemcpy(out, q->l_out, sizeof(float) * 4 * q->n_spectra, hipMemcpyDeviceToHost));
     return GX_SUCCESS;
 }
 
 namespace gx {
 
-// learned e-value stage of a batched search: top hits, features, MLP,
-// routing, fallback fits for the rest (captured into graphs like the search)
-hipError_t enqueue_learned_evalue(gx_ctx *c, gx_dqueries *q, EvalArgs ea, float conf, hipStream_t s) {
-    hipError_t e;
+// the learned e-value stage of a batched search: one fused kernel (captured
+// into graphs like the search)
+hipError_t enqueue_learned_evalue(gx_ctx *c, gx_dqueries *q, const EvalArgs &ea, float conf, hipStream_t s) {
     gx_mlp *M = static_cast<gx_mlp *>(c->surrogate);
-    const uint32_t ns = q->n_spectra;
-    ea.mode = 1;
-    if ((e = launch_evalue(ea, s))) return e;
-    const FeatArgs fa = feat_args(q);
-    if ((e = launch_features(fa, s))) return e;
-    if ((e = forward(M, q->l_feat, ns, s, false, q->l_h1, q->l_h2, nullptr, nullptr, q->l_out))) return e;
-    RouteArgs ra{};
-    ra.n = ns;
-    ra.out = q->l_out;
-    ra.sparse = q->l_sparse;
-    ra.anchor = q->l_anchor;
-    ra.norm = M->norm;
-    ra.conf_threshold = conf;
-    ra.expect_max = ea.expect_max;
-    ra.psm = q->psm;
-    ra.route = q->l_route;
-    ra.stats = c->d_stats;
-    if ((e = launch_route(ra, s))) return e;
-    ea.mode = 2;
-    ea.route = q->l_route;
-    return launch_evalue(ea, s);
+    LearnArgs la{};
+    la.n = q->n_spectra;
+    la.buckets = ea.buckets;
+    la.min_cpsm = ea.min_cpsm;
+    la.expect_max = ea.expect_max;
+    la.best_key = ea.best_key;
+    la.best_eq = ea.best_eq;
+    la.best_shp = ea.best_shp;
+    la.cpsms = ea.cpsms;
+    la.hist = ea.hist;
+    la.frag = M->frag;
+    la.params = M->w;
+    la.norm = M->norm;
+    // sigmoid(logit) >= conf  <=>  logit >= log(conf / (1 - conf))
+    la.conf_logit = conf <= 0.0f ? -INFINITY : conf >= 1.0f ? INFINITY : std::log(conf / (1.0f - conf));
+    la.out = ea.out;
+    la.raw = q->l_out;
+    la.stats = ea.stats;
+    return launch_learned_evalue(la, s);
 }
 
 int ensure_learned(gx_ctx *c, gx_dqueries *q) {
     if (!c->surrogate) return GX_ERR_NOT_READY;
-    const size_t ns = q->n_spectra;
-    int st = alloc_features(q);
-    if (st) return st;
-    if (!q->l_h1) {
+    if (!q->l_out) {
         q->buf_gen++;
-        if (dalloc(&q->l_h1, ns * kHidden) || dalloc(&q->l_h2, ns * kHidden) || dalloc(&q->l_out, ns * kOutPad) ||
-            dalloc(&q->l_route, ns))
-            return GX_ERR_BAD_MEM_ALLOC;
+        if (dalloc(&q->l_out, (size_t)q->n_spectra * 4)) return GX_ERR_BAD_MEM_ALLOC;
     }
     return GX_SUCCESS;
 }

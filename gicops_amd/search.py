@@ -204,21 +204,27 @@ class Searcher:
 
     def features(self, dq: DeviceQueries) -> dict:
         """Surrogate features and plain-fit targets of a searched batch
-        (gx_features_download): feat (n,128), anchor (n,), target (n,2) =
-        slope, u - anchor; loge (n,) = log10 of the plain e-value; valid,
-        sparse (n,) bool."""
+        (gx_features_download): feat (n,128), target (n,4) = {mark before the
+        mark2 rule, mark2, n-1, servable}; loge (n,) = log10 of the plain
+        e-value; valid, servable (n,) bool."""
         n = dq.n_spectra
         feat = np.zeros((n, F.MLP_FEAT), np.float32)
-        anchor = np.zeros(n, np.float32)
-        target = np.zeros((n, 2), np.float32)
+        target = np.zeros((n, F.MLP_TARGET), np.float32)
         loge = np.zeros(n, np.float32)
         valid = np.zeros(n, np.uint8)
-        sparse = np.zeros(n, np.uint8)
+        serv = np.zeros(n, np.uint8)
         F.check("gx_features_download", self._lib.gx_features_download(
-            self._ctx, dq._h, F.fptr(feat), F.fptr(anchor), F.fptr(target), F.fptr(loge), F.u8ptr(valid),
-            F.u8ptr(sparse)))
-        return {"feat": feat, "anchor": anchor, "target": target, "loge": loge, "valid": valid.astype(bool),
-                "sparse": sparse.astype(bool)}
+            self._ctx, dq._h, F.fptr(feat), F.fptr(target), F.fptr(loge), F.u8ptr(valid), F.u8ptr(serv)))
+        return {"feat": feat, "target": target, "loge": loge, "valid": valid.astype(bool),
+                "servable": serv.astype(bool)}
+
+    def surrogate_outputs(self, dq: DeviceQueries) -> np.ndarray:
+        """After a learned-mode search: (n,4) raw outputs {mark, mark2,
+        confidence logit} and 1.0 where the surrogate served the e-value."""
+        out = np.zeros((dq.n_spectra, 4), np.float32)
+        F.check("gx_surrogate_outputs_download",
+                self._lib.gx_surrogate_outputs_download(self._ctx, dq._h, F.fptr(out)))
+        return out
 
     def close(self) -> None:
         for ch in self._children:

@@ -2,14 +2,15 @@
 
 The plain e-value model is expeRT::ModelTailFit (expeRT.cpp:396-559): a
 least-squares line through the log10 survival of the candidate-score
-histogram between the 22% and 87% marks of the tail, evaluated at the top
-hit's bin. The surrogate is a 128-256-256-3 GELU MLP that maps the survival
-curve near the top bin to (slope, fitted log10 survival at the top bin,
-confidence). It is trained on the GPU from pairs that the plain fit itself
-produced; at search time (SearchParams.evalue_mode = GX_EVALUE_LEARNED) its
-value is used where it is confident and the histogram is not sparse, and the
-plain fit runs for every other modeled spectrum. Every call goes through the
-HIP library; there is no CPU fallback.
+histogram between two marks (expeRT.cpp:477-515), evaluated at the top hit's
+bin. The surrogate is a 128-256-256-3 GELU MLP that predicts where the two
+marks lie on the curve and how sure it is. It is trained on the GPU from the
+marks the plain fit itself finds. At search time (SearchParams.evalue_mode =
+GX_EVALUE_LEARNED) one fused kernel runs the network, checks each predicted
+mark against its defining inequality among the 32 points around it, and fits
+the line on the window in parallel; the plain fit runs for every modeled
+spectrum the surrogate does not serve. Every call goes through the HIP
+library; there is no CPU fallback.
 """
 from __future__ import annotations
 
@@ -28,13 +29,12 @@ class TrainParams:
     lr: float = 2e-3
     lr_decay: float = 0.7
     conf_weight: float = 0.5
-    tol: float = 1e-3
     seed: int = 0
 
     def to_ffi(self) -> F.gx_train_params:
         t = F.gx_train_params()
         t.batch, t.epochs, t.lr, t.lr_decay = self.batch, self.epochs, self.lr, self.lr_decay
-        t.conf_weight, t.tol, t.seed = self.conf_weight, self.tol, self.seed
+        t.conf_weight, t.seed = self.conf_weight, self.seed
         return t
 
 
@@ -81,22 +81,20 @@ class Surrogate:
         F.check("gx_mlp_train", self._lib.gx_mlp_train(self._h, C.byref(t), C.byref(rep)))
         e = rep.epochs
         return {"samples": rep.samples, "epochs": e,
-                "mse_slope": list(rep.mse_slope[:e]), "mse_u": list(rep.mse_u[:e]),
-                "bce": list(rep.bce[:e]), "within_tol": list(rep.within_tol[:e])}
+                "mse_mark": list(rep.mse_mark[:e]), "mse_mark2": list(rep.mse_mark2[:e]),
+                "bce": list(rep.bce[:e]), "in_window": list(rep.in_window[:e])}
 
-    def grad(self, feat, target, loge, norm, conf_weight: float, tol: float):
-        """Gradient of the mean loss on host rows -> (grad, out (n,16), loss sums[4])."""
+    def grad(self, feat, target, norm, conf_weight: float):
+        """Gradient of the mean loss on host rows (target n x 4) -> (grad, out (n,16), loss sums[4])."""
         feat = np.ascontiguousarray(feat, np.float32)
         target = np.ascontiguousarray(target, np.float32)
-        loge = np.ascontiguousarray(loge, np.float32)
         norm = np.ascontiguousarray(norm, np.float32)
         n = feat.shape[0]
         g = np.zeros(self.n, np.float32)
         out = np.zeros((n, F.MLP_OUT_PAD), np.float32)
         loss = np.zeros(4, np.float32)
-        F.check("gx_mlp_grad", self._lib.gx_mlp_grad(self._h, F.fptr(feat), F.fptr(target), F.fptr(loge), n,
-                                                     F.fptr(norm), conf_weight, tol, F.fptr(g), F.fptr(out),
-                                                     F.fptr(loss)))
+        F.check("gx_mlp_grad", self._lib.gx_mlp_grad(self._h, F.fptr(feat), F.fptr(target), n, F.fptr(norm),
+                                                     conf_weight, F.fptr(g), F.fptr(out), F.fptr(loss)))
         return g, out, loss
 
     def predict(self, feat) -> np.ndarray:

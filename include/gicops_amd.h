@@ -347,29 +347,35 @@ int gx_last_error(char *buf, size_t len);
 int gx_ctx_device_info(gx_ctx *ctx, char *name, size_t name_len, int *n_cu);
 
 /* ---- learned e-value surrogate ----------------------------------------
- * A 128-256-256-3 GELU MLP (BF16 operands on MFMA, FP32 accumulation and
- * master weights) that maps a spectrum's survival histogram to the tail-fit
- * model of expeRT::ModelTailFit (expeRT.cpp:396-559): slope, fitted
- * log10-survival at the top hit's bin, and a confidence logit. With
- * evalue_mode = GX_EVALUE_LEARNED the batched engine takes the surrogate's
- * e-value where the confidence is at least surrogate_conf, the histogram
- * has at least 3 non-empty bins below the top hit, both regression outputs
- * lie within 4 standard deviations of the training targets and the fitted
- * log10 survival at the top bin is <= 0; every other modeled
- * spectrum gets the plain tail fit (GX_PSM_SURROGATE tells them apart, and
- * stats.fallbacks counts the plain fits). Top hits, cpsms and histograms are
- * the same in both modes. No reference counterpart: the reference always
- * runs the plain fit.
+ * No reference counterpart: the reference always runs expeRT::ModelTailFit
+ * (expeRT.cpp:396-559), a least-squares line through a spectrum's log10
+ * survival curve between two marks, expeRT.cpp:477-515 (one before the first
+ * point at or below 22 % of the curve's drop, and the last point at or above
+ * 87 % of it), evaluated at the top hit's bin.
  *
- * Features (per spectrum, bf16): the plain fit's log10 survival curve
- * (bins stt1..end1, from the first non-empty bin to the last non-empty one
- * below the top hit, expeRT.cpp:414-475) resampled to 120 points by linear
- * interpolation and divided by 4, then stt1/800, end1/800, hyp/800,
- * (hyp-end1)/400, n/400, log10(cpsms)/4, anchor/4 and the first curve value,
- * where the anchor is the curve's last value. Targets: slope = mu/1e6 and
- * u - anchor with u = slope*hyp + beta/1e6 (the reference's truncated
- * integers); the surrogate's e-value is cpsms * 10^(anchor + predicted
- * residual). */
+ * The surrogate is a 128-256-256-3 GELU MLP (BF16 operands on MFMA, FP32
+ * accumulation and master weights) that predicts where those two marks lie,
+ * as fractions of the curve, and a confidence logit. With evalue_mode =
+ * GX_EVALUE_LEARNED the batched engine runs one fused kernel per 16 spectra:
+ * features, the network, then for each spectrum whose curve is servable (at
+ * least 8 points, no survival value <= 0) and whose confidence is at least
+ * surrogate_conf, each mark is looked for among 32 points around its
+ * predicted index; when both marks are found there, the line is fitted on
+ * that window with wave-parallel sums and the e-value formed from the
+ * truncated mu, beta as the plain path forms it (GX_PSM_SURROGATE set).
+ * Every other modeled spectrum gets the plain fit, bit-identical to
+ * GX_EVALUE_TAILFIT (stats.fallbacks counts them). Top hits, cpsms and
+ * histograms are the same in both modes. A served e-value uses the plain
+ * fit's own window; its mu, beta can differ from the plain fit's only through
+ * the summation order of the line fit.
+ *
+ * Features (per spectrum, bf16): the log10 survival 1 - C/cpsms, where C is
+ * the prefix count of the histogram from the first non-empty bin below the
+ * top hit (stt1) to the last (end1), linearly interpolated at 120 points
+ * t = j*(n-1)/119 and divided by 4; then stt1/800, end1/800, hyp/800,
+ * (hyp-end1)/400, n/400, log10(cpsms)/4, the last curve value/4 and the first
+ * curve value (single precision, log10 of max(x, 1e-7)). Training targets
+ * per row: {mark before the mark2 rule, mark2, n-1, servable}. */
 typedef struct gx_mlp gx_mlp;
 
 #define GX_TRAIN_MAX_EPOCHS 64
@@ -379,7 +385,7 @@ typedef struct gx_train_params {
     float lr;            /* Adam step size (b1 0.9, b2 0.999, eps 1e-8) */
     float lr_decay;      /* per-epoch multiplier of lr (0 or 1 = constant) */
     float conf_weight;   /* weight of the confidence cross-entropy */
-    float tol;           /* confidence label: |u_pred - u| <= tol * max(|log10 e|, 1e-3) */
+    int32_t pad;
     uint64_t seed;       /* minibatch shuffling */
 } gx_train_params;
 
@@ -387,12 +393,14 @@ typedef struct gx_train_report {
     uint64_t samples;    /* training rows */
     int32_t epochs;      /* epochs run */
     int32_t pad;
-    /* per epoch, means over the rows seen: squared errors of the normalised
-     * slope and u, confidence cross-entropy, fraction within tolerance */
-    float mse_slope[GX_TRAIN_MAX_EPOCHS];
-    float mse_u[GX_TRAIN_MAX_EPOCHS];
+    /* per epoch, means over the rows seen: squared errors of the two
+     * normalised mark positions, confidence cross-entropy, and the fraction
+     * of rows whose two marks lay in the check windows around the predicted
+     * indexes (the confidence label) */
+    float mse_mark[GX_TRAIN_MAX_EPOCHS];
+    float mse_mark2[GX_TRAIN_MAX_EPOCHS];
     float bce[GX_TRAIN_MAX_EPOCHS];
-    float within_tol[GX_TRAIN_MAX_EPOCHS];
+    float in_window[GX_TRAIN_MAX_EPOCHS];
 } gx_train_report;
 
 /* Number of fp32 parameters: W1[128][256], b1[256], W2[256][256], b2[256],
@@ -404,34 +412,39 @@ void gx_mlp_destroy(gx_mlp *m);
 /* Glorot-uniform weights, zero biases, Adam state cleared. */
 int gx_mlp_init(gx_mlp *m, uint64_t seed);
 /* Set / read the parameters; `norm` (optional, 4 floats) is the output
- * normalisation {slope shift, slope scale, u shift, u scale}: target =
- * output * scale + shift. Setting parameters clears the Adam state. */
+ * normalisation {mark shift, mark scale, mark2 shift, mark2 scale}: mark
+ * position (fraction of n-1) = output * scale + shift. Setting parameters
+ * clears the Adam state. */
 int gx_mlp_set_params(gx_mlp *m, const float *params, size_t n, const float *norm);
 int gx_mlp_get_params(gx_mlp *m, float *params, size_t n, float *norm);
 /* Append the spectra of a batch searched in GX_EVALUE_TAILFIT mode whose
- * plain fit is defined to the training set (features and targets stay in
- * HBM). */
+ * curve is servable and whose plain fit is defined to the training set
+ * (features and targets stay in HBM). */
 int gx_mlp_add_training(gx_mlp *m, gx_dqueries *q, uint64_t *n_added);
 /* Train on the GPU (forward, loss, backward and Adam kernels). The output
  * normalisation is fixed from the training targets on the first call unless
  * set by gx_mlp_set_params. */
 int gx_mlp_train(gx_mlp *m, const gx_train_params *tp, gx_train_report *rep);
-/* One forward + backward pass on host rows (feat n*128, target n*2, loge n)
- * with the given normalisation; writes the gradient of the mean loss
+/* One forward + backward pass on host rows (feat n*128, target n*4) with the
+ * given normalisation; writes the gradient of the mean loss
  * (gx_mlp_param_count floats), and optionally the raw outputs (n*16) and the
- * loss sums {sq. err slope, sq. err u, cross-entropy, rows within tol}. */
-int gx_mlp_grad(gx_mlp *m, const float *feat, const float *target, const float *loge, uint32_t n,
-                const float *norm, float conf_weight, float tol, float *grad, float *out, float *loss);
+ * loss sums {sq. err mark, sq. err mark2, cross-entropy, rows labelled 1}. */
+int gx_mlp_grad(gx_mlp *m, const float *feat, const float *target, uint32_t n, const float *norm,
+                float conf_weight, float *grad, float *out, float *loss);
 /* Forward pass on host rows: out = n*16 raw outputs (columns 0..2 used). */
 int gx_mlp_predict(gx_mlp *m, const float *feat, uint32_t n, float *out);
 /* Attach the surrogate used by GX_EVALUE_LEARNED searches (NULL detaches). */
 int gx_ctx_set_surrogate(gx_ctx *ctx, gx_mlp *m);
-/* Features (n*128), anchors (n), targets (n*2), log10 of the plain e-value
- * (n), and the valid / sparse flags of a searched batch (any output pointer
- * may be NULL). Targets and log10 e are meaningful for spectra searched with
- * GX_EVALUE_TAILFIT whose valid flag is set. */
-int gx_features_download(gx_ctx *ctx, gx_dqueries *q, float *feat, float *anchor, float *target, float *loge,
-                         uint8_t *valid, uint8_t *sparse);
+/* Features (n*128), targets (n*4), log10 of the plain e-value (n), and the
+ * valid (plain fit defined) / servable flags of a searched batch (any output
+ * pointer may be NULL). Targets and log10 e are those of the batch's last
+ * search, meaningful after a GX_EVALUE_TAILFIT search. */
+int gx_features_download(gx_ctx *ctx, gx_dqueries *q, float *feat, float *target, float *loge, uint8_t *valid,
+                         uint8_t *servable);
+/* After a GX_EVALUE_LEARNED search of `q`: per spectrum the network's raw
+ * outputs {mark, mark2, confidence logit} and 1.0 where the surrogate served
+ * the e-value, else 0.0 (n*4 floats). */
+int gx_surrogate_outputs_download(gx_ctx *ctx, gx_dqueries *q, float *out);
 
 #ifdef __cplusplus
 }

@@ -1,35 +1,41 @@
 """CPU restatement of the learned e-value surrogate - TEST INFRASTRUCTURE ONLY.
 
 Imported by tests/ and bench.py's accuracy report, never by the gicops_amd
-package. It restates, in numpy float64:
+package. It restates, in numpy:
 
-- the surrogate features of a searched spectrum: the log10 survival curve
-  that expeRT::ModelTailFit regresses on (expeRT.cpp:414-475), resampled to
-  120 points, and 8 scalars (curve bounds, top bin, length, candidate count,
-  the curve's last value as the anchor, its first value);
-- the targets: the plain fit's slope mu/1e6 and its fitted log10 survival at
-  the top bin u = mu/1e6*hyp + beta/1e6 (expeRT.cpp:526-559; the e-value is
-  cpsms * 10^u), the latter as a residual over the anchor;
-- the MLP forward pass, loss and backward pass, with the same bf16 rounding
-  points as the HIP kernels when `quantize=True` (inputs, weights, hidden
-  activations, output gradient and hidden-layer gradients are rounded to
-  bf16; accumulation, biases and master weights stay wide);
-- the confidence routing.
+- the tail fit's curve (expeRT.cpp:414-475): bounds stt1/end1 below the top
+  hit's bin, exact prefix counts, log10 survival in the plain fit's double
+  arithmetic;
+- the surrogate's features in the kernels' single-precision arithmetic (the
+  log10 survival of linearly interpolated prefix counts at 120 points, and 8
+  scalars), rounded to bf16;
+- the targets: the plain fit's marks (expeRT.cpp:477-515) - one before the
+  first point at or below 22 % of the curve's drop, the last point at or
+  above 87 % of it - the curve length and whether the surrogate may serve;
+- the MLP forward pass, loss (squared errors of the two normalised mark
+  positions, cross-entropy of the confidence against the window-check label)
+  and backward pass, with the same bf16 rounding points as the HIP kernels
+  when `quantize=True`;
+- the learned e-value stage: predicted indexes, the 32-point check windows
+  with the kernel's acceptance rules, the least-squares line on the window
+  (Burkardt's llsq, expeRT.cpp:1241-1289) and the e-value from the truncated
+  mu, beta (dslim_query.cpp:1023-1041).
 
-It is pinned by central finite differences of the float64 loss (backward vs
-forward), by GELU against the normal CDF, and by hand-computed features of
-small histograms (tests/test_mlp_oracle.py).
+It is pinned by central finite differences of the float64 loss, GELU against
+the normal CDF, bf16 against torch, hand-computed curves and features, the
+window fit against the plain tail-fit oracle (orc_tailfit) on random tails,
+and the closed-form window label against a brute-force scan of the windows
+(tests/test_mlp_oracle.py).
 """
 from __future__ import annotations
 
 import math
 
 import numpy as np
-from scipy.special import erf
 
 FEAT, HIDDEN, OUT_PAD, OUT, CURVE_PTS, HIST_BINS = 128, 256, 16, 3, 120, 802
-MIN_TAIL_BINS = 3
-PSM_MODELED, PSM_FIT_OK = 2, 4
+TARGET, MIN_CURVE, SNAP = 4, 8, 32
+PSM_MODELED, PSM_FIT_OK, PSM_REPORTED, PSM_SURROGATE = 2, 4, 8, 16
 
 # flat parameter layout (row-major blocks, in this order)
 W1 = 0
@@ -60,20 +66,25 @@ def pack(d: dict) -> np.ndarray:
     return np.concatenate([d["W1"].ravel(), d["b1"], d["W2"].ravel(), d["b2"], d["W3"].ravel(), d["b3"]])
 
 
+GELU_K, GELU_C = 2.0 * math.sqrt(2.0 / math.pi), 0.044715
+
+
 def gelu(z):
-    return 0.5 * z * (1.0 + erf(z / math.sqrt(2.0)))
+    """GELU, tanh form: 0.5 z (1 + tanh(sqrt(2/pi) (z + 0.044715 z^3)))
+    = z * sigmoid(2 sqrt(2/pi) (z + 0.044715 z^3))."""
+    return 0.5 * z * (1.0 + np.tanh(0.5 * GELU_K * (z + GELU_C * z ** 3)))
 
 
 def gelu_grad(z):
-    return 0.5 * (1.0 + erf(z / math.sqrt(2.0))) + z * np.exp(-0.5 * z * z) / math.sqrt(2.0 * math.pi)
+    s = 1.0 / (1.0 + np.exp(-GELU_K * (z + GELU_C * z ** 3)))
+    return s + z * s * (1.0 - s) * GELU_K * (1.0 + 3.0 * GELU_C * z * z)
 
 
-# ---- features -------------------------------------------------------------
-def curve(h: np.ndarray, hyp: int, cps: int):
-    """The plain fit's log10 survival curve (expeRT.cpp:414-475): bins stt1
-    (first non-empty) .. end1 (last non-empty below hyp; +1 when equal),
-    s = 1 - cumsum/cpsms from stt1, s > 1 -> 0.999, s <= 0 -> the last value
-    >= 1e-4 (or the last value), log10. Returns (stt1, end1, values, bad)."""
+# ---- curve and features -------------------------------------------------------
+def curve_prefix(h: np.ndarray, hyp: int):
+    """Curve bounds of the tail fit (expeRT.cpp:414-452) and exact prefix
+    counts: stt1 = first non-empty bin below hyp, end1 = last (hyp - 1 and 0
+    if none), end1 + 1 if they are equal; C[i] = sum h[stt1..stt1+i]."""
     nz = np.nonzero(h[:hyp])[0]
     end1 = int(nz[-1]) if len(nz) else hyp - 1
     stt1 = int(nz[0]) if len(nz) else 0
@@ -83,57 +94,179 @@ def curve(h: np.ndarray, hyp: int, cps: int):
     seg = np.zeros(n, np.int64)
     top = min(end1 + 1, HIST_BINS)
     seg[:top - stt1] = h[stt1:top]
-    sx = -(np.cumsum(seg) / cps - 1.0)
-    sx[sx > 1.0] = 0.999
-    good = np.nonzero(sx >= 1e-4)[0]
-    rv = sx[good[-1]] if len(good) else sx[n - 1]
-    sx = np.where(sx <= 0.0, rv, sx)
+    return stt1, end1, np.cumsum(seg)
+
+
+def log_survival(c, cps: int) -> np.ndarray:
+    """log10 of x = -(C/cpsms - 1) in the plain fit's order of operations
+    (expeRT.cpp:453-475), for curves with no replaced values."""
+    x = np.asarray(c, dtype=np.float64) / float(cps)
+    x = x + (-1.0)
+    x = -x
     with np.errstate(divide="ignore", invalid="ignore"):
-        ls = np.log10(sx)
-    bad = not np.all(np.isfinite(ls))
-    ls = np.where(np.isfinite(ls), ls, -8.0)
-    return stt1, end1, ls, bad
+        return np.log10(x)
+
+
+def _f32_log_survival(cnt, cps) -> np.float32:
+    x = np.float32(1.0) - np.float32(cnt) / np.float32(cps)
+    return np.log10(np.maximum(x, np.float32(1e-7))).astype(np.float32)
+
+
+def feature_row(c: np.ndarray, stt1: int, end1: int, hyp: int, cps: int) -> np.ndarray:
+    """The features of one modeled spectrum in the kernels' float32
+    arithmetic (before the bf16 rounding)."""
+    n = len(c)
+    f = np.zeros(FEAT, np.float32)
+    fc = np.float32(cps)
+    for j in range(CURVE_PTS):
+        t = np.float32(j * (n - 1)) / np.float32(CURVE_PTS - 1)
+        i0 = min(int(t), n - 1)
+        i1 = min(i0 + 1, n - 1)
+        fr = np.float32(t - np.float32(i0))
+        cnt = np.float32(np.float32(c[i0]) * (np.float32(1.0) - fr)) + np.float32(np.float32(c[i1]) * fr)
+        f[j] = _f32_log_survival(cnt, fc) / np.float32(4.0)
+    f[CURVE_PTS:] = [np.float32(stt1) / np.float32(800), np.float32(end1) / np.float32(800),
+                     np.float32(hyp) / np.float32(800), np.float32(hyp - end1) / np.float32(400),
+                     np.float32(n) / np.float32(400), np.log10(fc) / np.float32(4),
+                     _f32_log_survival(c[-1], fc) / np.float32(4), _f32_log_survival(c[0], fc)]
+    return f
+
+
+def servable_curve(c: np.ndarray, cps: int) -> bool:
+    """At least MIN_CURVE points and the last prefix count below cpsms (no
+    survival value <= 0, so none is replaced and the curve is monotone)."""
+    return len(c) >= MIN_CURVE and int(c[-1]) < cps
+
+
+def exact_marks(c: np.ndarray, cps: int):
+    """The plain fit's marks on a servable curve (expeRT.cpp:477-515):
+    (f, mark2, mark) with f the first point at or below the 22 % threshold
+    (None if none), mark2 the last point at or above the 87 % threshold and
+    mark = f - 1, moved to mark2 - 1 when not below mark2."""
+    ls = log_survival(c, cps)
+    n = len(ls)
+    s0 = ls[0]
+    hgt = ls[n - 1] - s0
+    t1, t2 = s0 + hgt * 0.22, s0 + hgt * 0.87
+    below = np.nonzero(ls <= t1)[0]
+    above = np.nonzero(ls >= t2)[0]
+    f = int(below[0]) if len(below) else None
+    mark2 = int(above[-1]) if len(above) else n - 1
+    mark = (f if f is not None else 0) - 1
+    if mark >= mark2:
+        mark = mark2 - 1
+    return f, mark2, mark
 
 
 def features(hist: np.ndarray, psm: np.ndarray):
-    """Features, anchors, targets, log10 e and the valid / sparse flags of
-    searched spectra (hist: n x 802 counts, psm: gx_psm records)."""
+    """Features (bf16, as float64), targets (n x 4: mark before the mark2
+    rule, mark2, n-1, servable), log10 of the plain e-value, and the valid /
+    servable flags of searched spectra (hist: n x 802, psm: gx_psm records)."""
     n = len(psm)
-    feat = np.zeros((n, FEAT), np.float64)
-    anchor = np.zeros(n, np.float64)
-    target = np.zeros((n, 2), np.float64)
-    loge = np.zeros(n, np.float64)
+    feat = np.zeros((n, FEAT), np.float32)
+    target = np.zeros((n, TARGET), np.float32)
+    loge = np.zeros(n, np.float32)
     valid = np.zeros(n, bool)
-    sparse = np.ones(n, bool)
+    serv = np.zeros(n, bool)
     for s in range(n):
         r = psm[s]
         hyp, cps, flags = int(r["hyp"]), max(int(r["cpsms"]), 0), int(r["flags"])
         modeled = bool(flags & PSM_MODELED) and cps > 0 and 0 < hyp < HIST_BINS
         fit = bool(flags & PSM_FIT_OK)
         u = int(r["mu"]) / 1e6 * hyp + int(r["beta"]) / 1e6
+        if fit and cps > 0:
+            loge[s] = np.float32(math.log10(cps) + u)
         if not modeled:
-            target[s] = (np.float32(int(r["mu"]) / 1e6), np.float32(u))
-            loge[s] = np.float32(math.log10(cps) + u) if fit and cps > 0 else 0.0
             continue
-        h = hist[s].astype(np.int64)
-        stt1, end1, ls, bad = curve(h, hyp, cps)
-        m = len(ls)
-        for j in range(CURVE_PTS):
-            t = j * (m - 1) / (CURVE_PTS - 1) if m > 1 else 0.0
-            i0 = min(int(t), m - 1)
-            i1 = min(i0 + 1, m - 1)
-            fr = t - i0
-            feat[s, j] = np.float32((ls[i0] * (1.0 - fr) + ls[i1] * fr) / 4.0)
-        lc = math.log10(cps)
-        feat[s, CURVE_PTS:] = [np.float32(stt1 / 800.0), np.float32(end1 / 800.0), np.float32(hyp / 800.0),
-                               np.float32((hyp - end1) / 400.0), np.float32(m / 400.0), np.float32(lc / 4.0),
-                               np.float32(ls[-1] / 4.0), np.float32(ls[0])]
-        anchor[s] = np.float32(ls[-1])
-        target[s] = (np.float32(int(r["mu"]) / 1e6), np.float32(u - ls[-1]))
-        loge[s] = np.float32(lc + u) if fit else 0.0
         valid[s] = fit
-        sparse[s] = bad or np.count_nonzero(h[:hyp]) < MIN_TAIL_BINS
-    return bf16(feat), anchor, target, loge, valid, sparse
+        stt1, end1, c = curve_prefix(hist[s].astype(np.int64), hyp)
+        feat[s] = feature_row(c, stt1, end1, hyp, cps)
+        mark = mark2 = 0
+        ok = False
+        if servable_curve(c, cps):
+            f, mark2, ruled = exact_marks(c, cps)
+            mark = (f if f is not None else 0) - 1
+            ok = f is not None and f >= 1 and ruled >= 0
+        target[s] = (mark, mark2, len(c) - 1, 1.0 if ok else 0.0)
+        serv[s] = ok
+    return bf16(feat), target, loge, valid, serv
+
+
+# ---- window check -----------------------------------------------------------------
+def predicted_index(o, scale, shift, n: int) -> int:
+    """Index of a mark from a raw output (float32, as the kernels compute it)."""
+    p = (np.float32(o) * np.float32(scale) + np.float32(shift)) * np.float32(n - 1)
+    if not (p > 0):
+        return 0
+    if p >= np.float32(n - 1):
+        return n - 1
+    return int(np.rint(p))
+
+
+def snap_base(q: int, n: int, back: int) -> int:
+    return min(max(q - back, 0), max(n - SNAP, 0))
+
+
+def snap_label(q1: int, q2: int, mark: int, mark2: int, n: int) -> bool:
+    """Both marks lie in their check windows (closed form of the scan in
+    `snap_windows`, for a monotone curve)."""
+    f = mark + 1
+    b1 = snap_base(q1, n, SNAP // 2 - 1)
+    ok1 = f < b1 + SNAP and (f > b1 or b1 == 0)
+    b2 = snap_base(q2, n, SNAP // 2)
+    e2 = min(b2 + SNAP, n)
+    ok2 = b2 <= mark2 < e2 and (mark2 < e2 - 1 or e2 == n)
+    return ok1 and ok2
+
+
+def snap_windows(ls: np.ndarray, q1: int, q2: int):
+    """The kernel's check on the curve itself: the first point at or below t1
+    found in the window around q1 (not on its first point unless that is the
+    curve's first), the last point at or above t2 in the window around q2 (not
+    on its last point unless that is the curve's last). Returns (mark, mark2)
+    after the mark2 rule, or None."""
+    n = len(ls)
+    s0 = ls[0]
+    hgt = ls[n - 1] - s0
+    t1, t2 = s0 + hgt * 0.22, s0 + hgt * 0.87
+    b1, b2 = snap_base(q1, n, SNAP // 2 - 1), snap_base(q2, n, SNAP // 2)
+    w1 = [p for p in range(b1, min(b1 + SNAP, n)) if ls[p] <= t1]
+    w2 = [p for p in range(b2, min(b2 + SNAP, n)) if ls[p] >= t2]
+    if not w1 or not w2:
+        return None
+    f = w1[0]
+    if (f == b1 and b1 > 0) or f == 0:
+        return None
+    mark2 = w2[-1]
+    e2 = min(b2 + SNAP, n)
+    if mark2 == e2 - 1 and e2 < n:
+        return None
+    mark = f - 1
+    if mark >= mark2:
+        mark = mark2 - 1
+    if mark < 0:
+        return None
+    return mark, mark2
+
+
+def window_fit(ls: np.ndarray, stt1: int, mark: int, mark2: int):
+    """Least-squares line of log10 survival over x = stt1+mark .. stt1+mark2
+    (expeRT.cpp:516-545), as (slope, intercept)."""
+    y = ls[mark:mark2 + 1]
+    x = np.arange(stt1 + mark, stt1 + mark2 + 1, dtype=np.float64)
+    m = len(y)
+    if m == 1:
+        return 0.0, float(y[0])
+    xbar, ybar = x.sum() / m, y.sum() / m
+    top = np.sum((x - xbar) * (y - ybar))
+    bot = np.sum((x - xbar) ** 2)
+    a = top / bot
+    return float(a), float(ybar - a * xbar)
+
+
+def evalue_of(mu: int, beta: int, hyp: int, cps: int) -> float:
+    """e = cpsms * 10^((mu/1e6) hyp + beta/1e6) (dslim_query.cpp:1023-1041)."""
+    return float(10.0 ** ((mu / 1e6) * hyp + beta / 1e6) * cps)
 
 
 # ---- network --------------------------------------------------------------
@@ -151,37 +284,43 @@ def forward(w: np.ndarray, x: np.ndarray, quantize: bool = True) -> dict:
     return {"x": xb, "W1": W1b, "W2": W2b, "W3": W3b, "z1": z1, "h1": h1, "z2": z2, "h2": h2, "out": out}
 
 
-def loss_terms(out: np.ndarray, target: np.ndarray, loge: np.ndarray, norm, conf_weight: float, tol: float,
-               y: np.ndarray | None = None):
+def loss_terms(out: np.ndarray, target: np.ndarray, norm, conf_weight: float, y: np.ndarray | None = None):
     """Per-row errors, confidence labels and the mean loss
-    mean(ea^2 + eu^2 + conf_weight * bce). `y` fixes the labels (for finite
-    differences); by default they are |u_pred - u| <= tol * max(|loge|, 1e-3)."""
+    mean(e0^2 + e1^2 + conf_weight * bce), e_k = output_k - (mark_k/(n-1) -
+    shift_k)/scale_k over servable rows (0 elsewhere). `y` fixes the labels
+    (for finite differences); by default a row is labelled 1 when it is
+    servable and both marks lie in the check windows around the indexes its
+    outputs predict."""
     a_shift, a_scale, u_shift, u_scale = (float(v) for v in norm)
-    ta = (target[:, 0] - a_shift) / a_scale
-    tu = (target[:, 1] - u_shift) / u_scale
-    ea = out[:, 0] - ta
-    eu = out[:, 1] - tu
+    n = len(out)
+    serv = target[:, 3] > 0.5
+    nm1 = np.where(serv, target[:, 2], 1.0)
+    e0 = np.where(serv, out[:, 0] - (target[:, 0] / nm1 - a_shift) / a_scale, 0.0)
+    e1 = np.where(serv, out[:, 1] - (target[:, 1] / nm1 - u_shift) / u_scale, 0.0)
     if y is None:
-        y = (np.abs(eu) * u_scale <= tol * np.maximum(np.abs(loge), 1e-3)).astype(np.float64)
+        y = np.zeros(n)
+        for i in np.nonzero(serv)[0]:
+            m = int(target[i, 2]) + 1
+            q1 = predicted_index(out[i, 0], a_scale, a_shift, m)
+            q2 = predicted_index(out[i, 1], u_scale, u_shift, m)
+            y[i] = 1.0 if snap_label(q1, q2, int(target[i, 0]), int(target[i, 1]), m) else 0.0
     p = 1.0 / (1.0 + np.exp(-out[:, 2]))
     bce = -(y * np.log(np.maximum(p, 1e-7)) + (1 - y) * np.log(np.maximum(1 - p, 1e-7)))
-    n = len(out)
-    mean_loss = float(np.sum(ea * ea + eu * eu + conf_weight * bce) / n)
-    return ea, eu, y, p, bce, mean_loss
+    mean_loss = float(np.sum(e0 * e0 + e1 * e1 + conf_weight * bce) / n)
+    return e0, e1, y, p, bce, mean_loss
 
 
-def backward(w: np.ndarray, x: np.ndarray, target: np.ndarray, loge: np.ndarray, norm, conf_weight: float,
-             tol: float, quantize: bool = True):
+def backward(w: np.ndarray, x: np.ndarray, target: np.ndarray, norm, conf_weight: float, quantize: bool = True):
     """Gradient of the mean loss w.r.t. the flat parameters, the raw outputs
-    and the loss sums {sq. err slope, sq. err u, bce, rows within tol}."""
+    and the loss sums {sq. err mark, sq. err mark2, bce, rows labelled 1}."""
     f = forward(w, x, quantize)
     q = bf16 if quantize else (lambda a: np.asarray(a, np.float64))
     out = f["out"]
     n = len(out)
-    ea, eu, y, p, bce, _ = loss_terms(out, target, loge, norm, conf_weight, tol)
+    e0, e1, y, p, bce, _ = loss_terms(out, target, norm, conf_weight)
     dout = np.zeros((n, OUT_PAD))
-    dout[:, 0] = 2.0 * ea / n
-    dout[:, 1] = 2.0 * eu / n
+    dout[:, 0] = 2.0 * e0 / n
+    dout[:, 1] = 2.0 * e1 / n
     dout[:, 2] = conf_weight * (p - y) / n
     dout = q(dout)
     g = {"W3": f["h2"].T @ dout, "b3": dout.sum(0)}
@@ -191,18 +330,47 @@ def backward(w: np.ndarray, x: np.ndarray, target: np.ndarray, loge: np.ndarray,
     dz1 = q((dz2 @ f["W2"].T) * gelu_grad(f["z1"]))
     g["W1"] = f["x"].T @ dz1
     g["b1"] = dz1.sum(0)
-    sums = np.array([np.sum(ea * ea), np.sum(eu * eu), np.sum(bce), np.sum(y)])
+    sums = np.array([np.sum(e0 * e0), np.sum(e1 * e1), np.sum(bce), np.sum(y)])
     return pack(g), out, sums
 
 
-def predict_evalue(out: np.ndarray, anchor: np.ndarray, psm: np.ndarray, norm):
-    """Surrogate e-value cpsms * 10^u (u = anchor + predicted residual) and
-    confidence of each row."""
+def serve(out: np.ndarray, hist: np.ndarray, psm: np.ndarray, norm, conf: float, expect_max: float = 20.0):
+    """The learned stage's decisions and served results from the network's
+    raw outputs (n x >=3) on spectra searched with the plain fit (`psm` gives
+    the top hits): per spectrum whether the surrogate serves it, and its mu,
+    beta, e-value and report flag where it does."""
     a_shift, a_scale, u_shift, u_scale = (float(v) for v in norm)
-    u = np.asarray(anchor, np.float64) + out[:, 1] * u_scale + u_shift
-    conf = 1.0 / (1.0 + np.exp(-out[:, 2]))
-    e = psm["cpsms"].astype(np.float64) * np.power(10.0, u)
-    return e, u, conf
+    logit = -np.inf if conf <= 0 else np.inf if conf >= 1 else math.log(conf / (1 - conf))
+    n = len(psm)
+    served = np.zeros(n, bool)
+    mu = np.zeros(n, np.int64)
+    beta = np.zeros(n, np.int64)
+    ev = np.full(n, np.inf)
+    rep = np.zeros(n, bool)
+    for s in range(n):
+        r = psm[s]
+        hyp, cps, flags = int(r["hyp"]), int(r["cpsms"]), int(r["flags"])
+        if not (flags & PSM_MODELED) or not 0 < hyp < HIST_BINS or not np.float32(out[s, 2]) >= np.float32(logit):
+            continue
+        stt1, end1, c = curve_prefix(hist[s].astype(np.int64), hyp)
+        if not servable_curve(c, cps):
+            continue
+        m = len(c)
+        q1 = predicted_index(out[s, 0], a_scale, a_shift, m)
+        q2 = predicted_index(out[s, 1], u_scale, u_shift, m)
+        ls = log_survival(c, cps)
+        w = snap_windows(ls, q1, q2)
+        if w is None:
+            continue
+        a, b = window_fit(ls, stt1, *w)
+        am, bm = a * 1e6, b * 1e6
+        if not (abs(am) < 2147483647.0 and abs(bm) < 2147483647.0):
+            continue
+        served[s] = True
+        mu[s], beta[s] = int(am), int(bm)
+        ev[s] = evalue_of(mu[s], beta[s], hyp, cps)
+        rep[s] = ev[s] < expect_max
+    return served, mu, beta, ev, rep
 
 
 def init_params(seed: int) -> np.ndarray:

@@ -1,7 +1,8 @@
 """The surrogate's CPU oracle, pinned without the GPU: GELU against the normal
 CDF, bf16 rounding against torch, the backward pass against central finite
-differences of the float64 loss, and the survival curve against hand-computed
-values and the tail-fit oracle (expeRT.cpp:414-559)."""
+differences of the float64 loss, the curve and features against hand-computed
+values, the window fit against the plain tail-fit oracle (expeRT.cpp:396-559),
+and the closed-form window label against a brute-force scan of the windows."""
 import math
 
 import numpy as np
@@ -16,14 +17,17 @@ from oracle import pyoracle as O
 
 def test_layout_matches_library_constants():
     assert M.N_PARAMS == 128 * 256 + 256 + 256 * 256 + 256 + 256 * 16 + 16
-    assert (M.FEAT, M.HIDDEN, M.OUT_PAD) == (F.MLP_FEAT, F.MLP_HIDDEN, F.MLP_OUT_PAD)
+    assert (M.FEAT, M.HIDDEN, M.OUT_PAD, M.TARGET) == (F.MLP_FEAT, F.MLP_HIDDEN, F.MLP_OUT_PAD, F.MLP_TARGET)
     w = np.arange(M.N_PARAMS, dtype=np.float64)
     assert np.array_equal(M.pack(M.unpack(w)), w)
 
 
-def test_gelu_is_z_times_normal_cdf():
+def test_gelu_is_torchs_tanh_gelu():
     z = np.linspace(-8, 8, 1001)
-    assert np.allclose(M.gelu(z), z * norm.cdf(z), rtol=1e-12, atol=1e-14)
+    ref = torch.nn.functional.gelu(torch.from_numpy(z), approximate="tanh").numpy()
+    assert np.allclose(M.gelu(z), ref, rtol=1e-12, atol=1e-14)
+    # within 1e-3 of the exact GELU z * Phi(z)
+    assert np.max(np.abs(M.gelu(z) - z * norm.cdf(z))) < 1e-3
     h = 1e-6
     fd = (M.gelu(z + h) - M.gelu(z - h)) / (2 * h)
     assert np.allclose(M.gelu_grad(z), fd, rtol=1e-6, atol=1e-8)
@@ -49,22 +53,25 @@ def _problem(n, seed):
     p["b3"][:M.OUT] = rng.normal(0, 0.1, M.OUT)
     w = M.pack(p)
     x = rng.normal(0, 0.5, (n, M.FEAT))
-    target = np.stack([rng.normal(-0.03, 0.01, n), rng.normal(-2.0, 1.0, n)], 1)
-    loge = rng.normal(-1.0, 2.0, n)
-    normv = np.array([-0.03, 0.01, -2.0, 1.0])
-    return w, x, target, loge, normv
+    m = rng.integers(20, 400, n)
+    mark = (rng.random(n) * 0.5 * (m - 1)).astype(int)
+    mark2 = np.maximum(mark + 1, ((0.7 + 0.3 * rng.random(n)) * (m - 1)).astype(int))
+    serv = rng.random(n) < 0.8
+    target = np.stack([mark, mark2, m - 1, serv.astype(float)], 1).astype(np.float64)
+    normv = np.array([0.25, 0.15, 0.95, 0.05])
+    return w, x, target, normv
 
 
 def test_backward_matches_finite_differences():
-    n, cw, tol = 24, 0.7, 5e-2
-    w, x, target, loge, normv = _problem(n, 3)
-    g, out, _ = M.backward(w, x, target, loge, normv, cw, tol, quantize=False)
+    n, cw = 24, 0.7
+    w, x, target, normv = _problem(n, 3)
+    g, out, _ = M.backward(w, x, target, normv, cw, quantize=False)
     # labels held fixed, as the gradient treats them
-    _, _, y, _, _, _ = M.loss_terms(out, target, loge, normv, cw, tol)
+    _, _, y, _, _, _ = M.loss_terms(out, target, normv, cw)
 
     def loss(wv):
         o = M.forward(wv, x, quantize=False)["out"]
-        return M.loss_terms(o, target, loge, normv, cw, tol, y=y)[-1]
+        return M.loss_terms(o, target, normv, cw, y=y)[-1]
 
     rng = np.random.default_rng(5)
     blocks = [(M.W1, M.B1), (M.B1, M.W2), (M.W2, M.B2), (M.B2, M.W3), (M.B3, M.B3 + M.OUT)]
@@ -84,9 +91,9 @@ def test_backward_matches_finite_differences():
 
 
 def test_quantized_backward_is_close_to_exact():
-    w, x, target, loge, normv = _problem(64, 9)
-    g32, _, _ = M.backward(w, x, target, loge, normv, 0.5, 1e-2, quantize=True)
-    g64, _, _ = M.backward(w, x, target, loge, normv, 0.5, 1e-2, quantize=False)
+    w, x, target, normv = _problem(64, 9)
+    g32, _, _ = M.backward(w, x, target, normv, 0.5, quantize=True)
+    g64, _, _ = M.backward(w, x, target, normv, 0.5, quantize=False)
     assert np.linalg.norm(g32 - g64) / np.linalg.norm(g64) < 0.05
 
 
@@ -96,65 +103,137 @@ def _psm(hyp, cps, flags=2 | 4, mu=0, beta=0):
     return r
 
 
-def test_features_of_a_hand_histogram():
+def test_curve_and_features_of_a_hand_histogram():
     h = np.zeros((1, M.HIST_BINS), np.uint32)
-    h[0, [3, 4, 6, 9]] = [4, 3, 2, 1]  # cpsms 10, the top hit alone in bin 9
-    feat, anchor, target, loge, valid, sparse = M.features(h, _psm(9, 10, mu=-250000, beta=-500000))
-    # curve: bins 3..6, s = 1 - cumsum/10 = 0.6, 0.3, 0.3, 0.1
-    ls = np.log10([0.6, 0.3, 0.3, 0.1])
-    assert anchor[0] == np.float32(ls[-1])
-    t = np.arange(M.CURVE_PTS) * 3 / (M.CURVE_PTS - 1)
-    assert np.array_equal(feat[0, :M.CURVE_PTS], M.bf16((np.interp(t, np.arange(4), ls) / 4).astype(np.float32)))
-    sc = [3 / 800, 6 / 800, 9 / 800, 3 / 400, 4 / 400, 1 / 4, ls[-1] / 4, ls[0]]
-    assert np.array_equal(feat[0, M.CURVE_PTS:], M.bf16(np.array(sc, np.float32)))
-    u = -0.25 * 9 - 0.5
-    assert target[0, 0] == np.float32(-0.25) and target[0, 1] == np.float32(u - ls[-1])
-    assert loge[0] == np.float32(1.0 + u)
-    assert valid[0] and not sparse[0]
-    # two non-empty bins below the top hit: sparse
+    # bins 3..12 below the top hit (bin 20): counts give survival 1 - C/40
+    h[0, 3:13] = [8, 6, 5, 0, 4, 3, 3, 2, 1, 1]
+    h[0, 20] = 7  # the top hit's bin and 6 more candidates there: cpsms 40
+    stt1, end1, c = M.curve_prefix(h[0].astype(np.int64), 20)
+    assert (stt1, end1) == (3, 12)
+    assert c.tolist() == [8, 14, 19, 19, 23, 26, 29, 31, 32, 33]
+    assert M.servable_curve(c, 40)
+    ls = M.log_survival(c, 40)
+    assert np.allclose(ls, np.log10(1 - c / 40), rtol=0, atol=1e-15)
+    # marks: t1 = ls0 + 0.22 (lsN - ls0), t2 = ls0 + 0.87 (lsN - ls0)
+    t1 = ls[0] + (ls[-1] - ls[0]) * 0.22
+    t2 = ls[0] + (ls[-1] - ls[0]) * 0.87
+    f = int(np.nonzero(ls <= t1)[0][0])
+    mark2 = int(np.nonzero(ls >= t2)[0][-1])
+    assert M.exact_marks(c, 40) == (f, mark2, f - 1)
+    feat, target, loge, valid, serv = M.features(h, _psm(20, 40, mu=-250000, beta=-500000))
+    assert valid[0] and serv[0]
+    assert target[0].tolist() == [f - 1, mark2, 9, 1.0]
+    assert loge[0] == np.float32(math.log10(40) - 0.25 * 20 - 0.5)
+    # curve points: linear interpolation of the prefix counts at t = j*9/119
+    t = np.arange(M.CURVE_PTS) * 9 / (M.CURVE_PTS - 1)
+    cnt = np.interp(t, np.arange(10), c)
+    assert np.allclose(feat[0, :M.CURVE_PTS], np.log10(1 - cnt / 40) / 4, rtol=2 ** -7, atol=1e-6)
+    sc = [3 / 800, 12 / 800, 20 / 800, 8 / 400, 10 / 400, math.log10(40) / 4, math.log10(1 - 33 / 40) / 4,
+          math.log10(1 - 8 / 40)]
+    assert np.allclose(feat[0, M.CURVE_PTS:], sc, rtol=2 ** -7, atol=1e-7)
+    # too short a curve: not servable (the plain fit serves it)
     h2 = np.zeros((1, M.HIST_BINS), np.uint32)
-    h2[0, [3, 6, 9]] = [5, 4, 1]
-    assert M.features(h2, _psm(9, 10))[5][0]
-    # not modeled: no features, sparse
-    f3 = M.features(h, _psm(9, 10, flags=1))
-    assert not f3[0].any() and f3[5][0] and not f3[4][0]
+    h2[0, [3, 5, 6, 9]] = [4, 3, 2, 1]
+    assert not M.features(h2, _psm(9, 10))[4][0]
+    # not modeled: no features
+    f3 = M.features(h, _psm(20, 40, flags=1))
+    assert not f3[0].any() and not f3[4][0] and not f3[3][0]
 
 
-def test_curve_agrees_with_the_tail_fit_oracle():
-    """With two curve bins the fit's line passes through both points
-    (mark 0, mark2 1), so its slope is their difference and its value at end1
-    is the anchor."""
+def _random_tail(rng):
     h = np.zeros(M.HIST_BINS, np.uint32)
-    h[[10, 11, 30]] = [5, 3, 1]
-    ok, slope, icept, mu, beta = O.tailfit(h, 30, 9)
-    assert ok
-    stt1, end1, ls, bad = M.curve(h.astype(np.int64), 30, 9)
-    assert (stt1, end1, bad) == (10, 11, False)
-    assert math.isclose(slope, ls[1] - ls[0], rel_tol=1e-12)
-    assert math.isclose(slope * end1 + icept, ls[-1], rel_tol=1e-12, abs_tol=1e-12)
-    # and on random tails, the fitted line at end1 stays within the curve's range
+    hyp = int(rng.integers(60, 700))
+    k = int(rng.integers(10, hyp - 1))
+    lo = int(rng.integers(0, k - 8))
+    h[lo:k] = rng.geometric(rng.uniform(0.05, 0.6), k - lo) * (rng.random(k - lo) < rng.uniform(0.3, 1.0))
+    h[hyp] = int(rng.integers(1, 4))
+    return h, hyp, int(h.sum())
+
+
+def test_window_fit_equals_the_tail_fit_oracle():
+    """On servable curves the exact marks and the window's least-squares line
+    give the plain tail fit's slope and intercept (orc_tailfit, a separate C
+    restatement of expeRT::ModelTailFit): the marks, thresholds and window
+    bounds agree with the reference's."""
     rng = np.random.default_rng(1)
-    for _ in range(200):
-        h = np.zeros(M.HIST_BINS, np.uint32)
-        hyp = int(rng.integers(60, 400))
-        k = int(rng.integers(5, hyp - 1))
-        h[:k] = rng.geometric(0.3, k) * (rng.random(k) < 0.7)
-        h[hyp] = 1
-        cps = int(h.sum())
-        ok, slope, icept, _, _ = O.tailfit(h, hyp, cps)
-        if not ok or np.count_nonzero(h[:hyp]) < 3:
+    checked = 0
+    for _ in range(600):
+        h, hyp, cps = _random_tail(rng)
+        stt1, end1, c = M.curve_prefix(h.astype(np.int64), hyp)
+        if not M.servable_curve(c, cps):
             continue
-        stt1, end1, ls, bad = M.curve(h.astype(np.int64), hyp, cps)
-        assert not bad
-        v = slope * end1 + icept
-        assert ls.min() - 1.0 <= v <= ls.max() + 1.0
+        f, mark2, mark = M.exact_marks(c, cps)
+        if f is None or f < 1 or mark < 0:
+            continue
+        ok, slope, icept, mu, beta = O.tailfit(h, hyp, cps)
+        assert ok
+        a, b = M.window_fit(M.log_survival(c, cps), stt1, mark, mark2)
+        assert math.isclose(a, slope, rel_tol=1e-10, abs_tol=1e-13), (a, slope)
+        assert math.isclose(b, icept, rel_tol=1e-10, abs_tol=1e-12), (b, icept)
+        assert abs(int(a * 1e6) - mu) <= 1 and abs(int(b * 1e6) - beta) <= 1
+        checked += 1
+    assert checked > 300
 
 
-def test_predicted_evalue_closed_form():
-    out = np.zeros((2, M.OUT_PAD))
-    out[:, 1] = [0.5, -1.0]
-    out[:, 2] = [0.0, 100.0]
-    e, u, conf = M.predict_evalue(out, np.array([-2.0, -3.0]), _psm(9, 100).repeat(2), [0, 1, -1.0, 2.0])
-    assert np.allclose(u, [-2.0 + 0.0, -3.0 - 3.0])
-    assert np.allclose(e, 100 * 10.0 ** u)
-    assert np.allclose(conf, [0.5, 1.0])
+def test_window_label_equals_the_window_scan():
+    """snap_label (the training label, a closed form) agrees with the scan the
+    kernel does on the curve (snap_windows) for every predicted index pair on
+    random servable curves, and a successful scan returns the exact marks."""
+    rng = np.random.default_rng(2)
+    checked = 0
+    for _ in range(200):
+        h, hyp, cps = _random_tail(rng)
+        stt1, end1, c = M.curve_prefix(h.astype(np.int64), hyp)
+        if not M.servable_curve(c, cps):
+            continue
+        f, mark2, mark = M.exact_marks(c, cps)
+        if f is None or f < 1 or mark < 0:
+            continue
+        n = len(c)
+        ls = M.log_survival(c, cps)
+        qs = sorted(set([0, n - 1, f - 1, f, mark2, max(0, mark2 - 20)] + list(rng.integers(0, n, 12))))
+        for q1 in qs:
+            for q2 in qs:
+                got = M.snap_windows(ls, q1, q2)
+                assert (got is not None) == M.snap_label(q1, q2, f - 1, mark2, n), (n, q1, q2, f, mark2)
+                if got is not None:
+                    assert got == (mark, mark2)
+                checked += 1
+    assert checked > 5000
+
+
+def test_predicted_index_rounds_to_nearest_and_clamps():
+    assert M.predicted_index(0.0, 1.0, 0.5, 101) == 50
+    assert M.predicted_index(0.0, 1.0, 0.505, 101) == 50  # 50.5 rounds to even
+    # float32(0.515) * 100 rounds to 51.5 in float32, and 51.5 to the even 52
+    assert M.predicted_index(0.0, 1.0, 0.515, 101) == 52
+    assert M.predicted_index(0.5, 0.02, 0.5, 101) == 51  # (0.5 * 0.02 + 0.5) * 100
+    assert M.predicted_index(-9.0, 1.0, 0.0, 101) == 0
+    assert M.predicted_index(9.0, 1.0, 0.0, 101) == 100
+    assert M.predicted_index(float("nan"), 1.0, 0.0, 101) == 0
+
+
+def test_serve_with_outputs_at_the_true_marks():
+    """Outputs placed on a curve's true marks are served with the plain fit's
+    window: mu and beta within one unit of orc_tailfit's, the e-value from the
+    truncated integers. A low confidence logit or a short curve is not
+    served."""
+    rng = np.random.default_rng(3)
+    for _ in range(100):
+        h, hyp, cps = _random_tail(rng)
+        stt1, end1, c = M.curve_prefix(h.astype(np.int64), hyp)
+        if not M.servable_curve(c, cps):
+            continue
+        f, mark2, mark = M.exact_marks(c, cps)
+        if f is None or f < 1 or mark < 0:
+            continue
+        n = len(c)
+        psm = _psm(hyp, cps, flags=2)
+        out = np.array([[(f - 1) / (n - 1), mark2 / (n - 1), 5.0]])
+        served, mu, beta, ev, rep = M.serve(out, h[None, :], psm, [0.0, 1.0, 0.0, 1.0], 0.9)
+        assert served[0]
+        ok, slope, icept, pmu, pbeta = O.tailfit(h, hyp, cps)
+        assert abs(mu[0] - pmu) <= 1 and abs(beta[0] - pbeta) <= 1
+        assert ev[0] == M.evalue_of(int(mu[0]), int(beta[0]), hyp, cps)
+        out[0, 2] = -5.0  # confidence 0.0067 < 0.9
+        assert not M.serve(out, h[None, :], psm, [0.0, 1.0, 0.0, 1.0], 0.9)[0][0]
