@@ -226,7 +226,7 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="open2m", choices=["open2m", "uneven", "small", "c4"],
                     help="c4 is 1M spectra of the uneven mix on the 50M-peptide index; run it with --steps 2 "
-                         "--warmup 0 --evalue plain --no-compare-spectrum --config 0 (about 2 minutes)")
+                         "--warmup 0 --no-compare-spectrum --config 0 (about 3 minutes)")
     ap.add_argument("--spectra", type=int, default=0, help="override the batch size")
     ap.add_argument("--cpu-budget", type=float, default=8.0, help="seconds of CPU baseline work per run (3 runs)")
     ap.add_argument("--no-cpu", action="store_true")
@@ -235,8 +235,9 @@ def main() -> int:
     ap.add_argument("--no-compare-spectrum", action="store_true",
                     help="skip the one run of the per-spectrum engine with the plain fit")
     ap.add_argument("--evalue", default="learned", choices=["learned", "plain"])
-    ap.add_argument("--train-spectra", type=int, default=500_000,
-                    help="spectra searched (held-out seeds) to make surrogate training pairs")
+    ap.add_argument("--train-spectra", type=int, default=0,
+                    help="spectra searched (held-out seeds) to make surrogate training pairs "
+                         "(0: 500k, or 100k for the uneven and 50M-peptide workloads)")
     ap.add_argument("--train-epochs", type=int, default=6)
     ap.add_argument("--surrogate-conf", type=float, default=0.5,
                     help="confidence threshold: below it the plain fit runs without a window check")
@@ -249,6 +250,8 @@ def main() -> int:
     args = ap.parse_args()
     if args.calibrate_copy:
         return calibrate_copy(args.calibrate_copy)
+    if not args.train_spectra:
+        args.train_spectra = 100_000 if args.workload in ("c4", "uneven") else 500_000
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for {args.gpus}); on more GPUs the search "
               "would shard by spectrum over a replicated index, with no communication until results are "
@@ -324,7 +327,7 @@ def main() -> int:
         # the e-value stage's device time (HIP events around it), eager runs of
         # the two forms alternated, median of 3 each
         ev_l, ev_p = [], []
-        for _ in range(3):
+        for _ in range(3 if batch.n_spectra <= 200_000 else 1):
             st = s.search_device(dq, params)
             ev_l.append(st["evalue_ms"])
             ev_p.append(s.search_device(dq, plain)["evalue_ms"])

@@ -26,7 +26,7 @@ from test_gpu_parity import compare  # noqa: E402
 
 pytestmark = pytest.mark.gpu
 
-SAMPLE = 300
+SAMPLE = 2000
 
 
 def say(msg):

@@ -6,8 +6,13 @@ builds the fragment-ion arrays (gx_index_build).
 Checked: every bucket within the engine's limits (< 2^26 entries, < 2^31
 ions); 100k spectra of the uneven mix (20-2000 raw peaks, QALEN 2048, dM
 drawn from {10, 100, 500} Da) searched in one call, with the size-independent
-properties over every spectrum; a seeded sample against the CPU oracle on a
-host copy of the GPU-built arrays, to the parity bar, with the work counters.
+properties over every spectrum; a seeded sample of 300 against the CPU oracle
+on a host copy of the GPU-built arrays, to the parity bar, with the work
+counters; and the learned e-value on the same batch, with a surrogate trained
+on held-out spectra of this index: top hits and histograms identical to the
+plain search, every e-value the surrogate did not serve bit-identical, every
+served one within 1e-3 relative of the plain fit's log10 e, and the fallback
+rate reported.
 """
 import dataclasses
 import sys
@@ -16,8 +21,10 @@ import time
 import numpy as np
 import pytest
 
+from gicops_amd import _ffi as F
 from gicops_amd import slm
 from gicops_amd.search import Searcher, SearchParams
+from gicops_amd.surrogate import Surrogate, TrainParams
 from oracle import pyoracle as O
 
 from test_gpu_fullsize import _invariants  # noqa: E402
@@ -25,7 +32,7 @@ from test_gpu_parity import compare  # noqa: E402
 
 pytestmark = pytest.mark.gpu
 
-SAMPLE = 48
+SAMPLE = 300
 
 
 def say(msg):
@@ -57,6 +64,43 @@ def test_c4_limits(c4):
     assert max(b.n_ions for b in index.buckets) < 2**31
 
 
+def _learned_leg(s, index, sc, dq, psm, hist):
+    """The learned e-value on the 50M-peptide batch: a surrogate trained on
+    held-out spectra of the same index, judged by the plain fit."""
+    sur = Surrogate(s, seed=5)
+    rows = 0
+    for seed in (4401, 4402, 4403):
+        b = slm.generate_spectra(index, dataclasses.replace(sc, n_spectra=20_000, seed=seed))
+        tq = s.upload(b)
+        s.search_device(tq, SearchParams(dM=500.0))
+        rows += sur.add_training(tq)
+        tq.free()
+    rep = sur.train(TrainParams(batch=1024, epochs=6, lr=2e-3, lr_decay=0.7, conf_weight=0.5, seed=1))
+    sur.attach()
+    lst = s.search_device(dq, SearchParams(dM=500.0, evalue_mode=F.GX_EVALUE_LEARNED, surrogate_conf=0.5))
+    lp, lh = s.download(dq, want_hist=True)
+    assert np.array_equal(lh, hist)
+    for k in ("hyperscore", "psid", "idxoffset", "sharedions", "totalions", "cpsms", "hyp"):
+        assert np.array_equal(lp[k], psm[k]), k
+    sur_m = (lp["flags"] & F.GX_PSM_SURROGATE) != 0
+    modeled = (psm["flags"] & F.GX_PSM_MODELED) != 0
+    rest = ~sur_m
+    for k in ("mu", "beta", "flags"):
+        assert np.array_equal(lp[k][rest], psm[k][rest]), k
+    assert np.array_equal(lp["evalue"][rest].view(np.uint64), psm["evalue"][rest].view(np.uint64))
+    assert np.all((psm["flags"][sur_m] & F.GX_PSM_FIT_OK) != 0)
+    le_ref, le = np.log10(psm["evalue"][sur_m]), np.log10(lp["evalue"][sur_m])
+    rel = np.abs(le - le_ref) / np.maximum(np.abs(le_ref), 1e-3)
+    assert np.all(rel <= 1e-3)
+    assert lst["fallbacks"] == int(modeled.sum() - sur_m.sum())
+    say(f"  learned: {rows} training rows, in window {rep['in_window'][-1]:.3f}; served {int(sur_m.sum())} of "
+        f"{int(modeled.sum())} modeled ({sur_m.sum() / max(1, modeled.sum()):.3f}), fallback rate "
+        f"{lst['fallbacks'] / max(1, modeled.sum()):.3f}, e-value stage {lst['evalue_ms']:.2f} ms, "
+        f"max rel. log10 e error {rel.max() if len(rel) else 0.0:.1e}")
+    assert sur_m.sum() > 0.3 * modeled.sum()
+    sur.close()
+
+
 def test_c4_search_sample_parity(c4):
     index, s = c4
     sc = dataclasses.replace(slm.SPEC_PRESETS["c4"], n_spectra=100_000)
@@ -70,6 +114,7 @@ def test_c4_search_sample_parity(c4):
         f"{st['chunks']} chunks, {st['ions']} ions, errors {st['errors']}")
     assert st["spectra"] == batch.n_spectra and st["errors"] == 0 and st["ions_scanned"] == st["ions"]
     _invariants(psm, hist)
+    _learned_leg(s, index, sc, dq, psm, hist)
     dq.free()
     say("  host copy of the GPU-built arrays for the oracle")
     host = s.host_index()
