@@ -18,78 +18,105 @@ __device__ __forceinline__ float gelu(float z) { return mlp_gelu(z); }
 
 __device__ __forceinline__ float gelu_grad(float z) { return mlp_gelu_grad(z); }
 
-// One wave per 16 x (16*NT) output strip; K in steps of 32. Lane l supplies
-// A[row l%16][k0 + 8*(l/16) .. +7] and B[k0 + 8*(l/16) .. +7][col l%16] and
-// receives C[4*(l/16) + j][l%16], j < 4 (the 16x16x32 bf16 MFMA layout; the
-// sum over k does not depend on which lane holds which k, as long as A and B
-// agree). The A fragment is loaded once per k step for all NT column tiles.
-// Out-of-range rows, columns and k are zero-filled, so any M, N, K work.
-template <int NT>
-__global__ void __launch_bounds__(256) gemm_kernel(GemmArgs g) {
-    const int lane = threadIdx.x & 63;
-    const int wave = threadIdx.x >> 6;
-    const int tiles_n = (g.N + 15) >> 4;
-    const int groups_n = (tiles_n + NT - 1) / NT;
-    const long long strip = (long long)blockIdx.x * 4 + wave;
-    const int tm = (int)(strip / groups_n), tg = (int)(strip % groups_n);
-    if ((long long)tm * 16 >= g.M) return;
-    const int r = lane & 15, kg = lane >> 4;
-    const int m = tm * 16 + r;
-    // 16-byte loads of A when its k run is contiguous and aligned
-    const bool a_vec = g.sak == 1 && (g.sam & 7) == 0 && (g.K & 31) == 0 && ((uintptr_t)g.A & 15) == 0;
-    floatx4 acc[NT];
-#pragma unroll
-    for (int t = 0; t < NT; t++) acc[t] = floatx4{0.0f, 0.0f, 0.0f, 0.0f};
-    for (int k0 = 0; k0 < g.K; k0 += 32) {
-        bf16x8 va;
-        if (a_vec) {
-            if (m < g.M) {
-                va = *reinterpret_cast<const bf16x8 *>(g.A + (long long)m * g.sam + k0 + 8 * kg);
-            } else {
-#pragma unroll
-                for (int j = 0; j < 8; j++) va[j] = (bf16)0.0f;
-            }
+// C = A B on MFMA with LDS tiles: one block of 4 waves per 64 x 64 output
+// tile, K in steps of 32. Each step stages the 64 x 32 A tile and the 32 x 64
+// B tile in LDS with 16-byte loads along whichever dimension of each operand
+// is contiguous (row-major, or transposed for the backward pass), stored as
+// [row][k] and [col][k] so that every MFMA fragment is one 16-byte LDS read:
+// lane l supplies A[row l%16][k0 + 8*(l/16) .. +7] and B[k0 + 8*(l/16) ..
+// +7][col l%16] and receives C[4*(l/16) + j][l%16], j < 4. Each wave computes
+// a 32 x 32 quarter (2 x 2 MFMA tiles). Elements outside M, N, K are zero, so
+// any shape works; the epilogue modes are those of GemmArgs.
+constexpr int kGT = 64;        // block tile
+constexpr int kGK = 32;        // k step
+constexpr int kGPad = kGK + 8; // LDS row stride (bf16): 80 B spreads a fragment read over the banks
+
+__device__ __forceinline__ void gemm_stage(const bf16 *P, long long s_row, long long s_k, int rows, int K, int r0,
+                                           int k0, bf16 (*T)[kGPad], int tid) {
+    // T[r][k] = P(r0 + r, k0 + k), 64 x 32
+    if (s_k == 1) {
+        // 8 consecutive k of one row per thread
+        const int r = tid >> 2, kc = (tid & 3) * 8;
+        const int gr = r0 + r, gk = k0 + kc;
+        const bf16 *src = P + (long long)gr * s_row + gk;
+        bf16x8 v;
+        if (gr < rows && gk + 8 <= K && (((uintptr_t)src) & 15) == 0) {
+            v = *reinterpret_cast<const bf16x8 *>(src);
         } else {
 #pragma unroll
-            for (int j = 0; j < 8; j++) {
-                const int k = k0 + 8 * kg + j;
-                va[j] = (k < g.K && m < g.M) ? g.A[(long long)m * g.sam + (long long)k * g.sak] : (bf16)0.0f;
-            }
+            for (int j = 0; j < 8; j++) v[j] = (gr < rows && gk + j < K) ? src[j] : (bf16)0.0f;
+        }
+        *reinterpret_cast<bf16x8 *>(&T[r][kc]) = v;
+    } else {
+        // 8 consecutive rows of one k per thread (s_row == 1 or general strides)
+        const int k = tid >> 3, rc = (tid & 7) * 8;
+        const int gk = k0 + k, gr = r0 + rc;
+        const bf16 *src = P + (long long)gr * s_row + (long long)gk * s_k;
+        bf16x8 v;
+        if (s_row == 1 && gk < K && gr + 8 <= rows && (((uintptr_t)src) & 15) == 0) {
+            v = *reinterpret_cast<const bf16x8 *>(src);
+        } else {
+#pragma unroll
+            for (int j = 0; j < 8; j++) v[j] = (gk < K && gr + j < rows) ? src[(long long)j * s_row] : (bf16)0.0f;
         }
 #pragma unroll
-        for (int t = 0; t < NT; t++) {
-            const int n = (tg * NT + t) * 16 + r;
-            bf16x8 vb;
+        for (int j = 0; j < 8; j++) T[rc + j][k] = v[j];
+    }
+}
+
+__global__ void __launch_bounds__(256) gemm_kernel(GemmArgs g) {
+    __shared__ __align__(16) bf16 As[kGT][kGPad];
+    __shared__ __align__(16) bf16 Bs[kGT][kGPad];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int tiles_n = (g.N + kGT - 1) / kGT;
+    const int m0 = (blockIdx.x / tiles_n) * kGT, n0 = (blockIdx.x % tiles_n) * kGT;
+    const int wm = (wave >> 1) * 32, wn = (wave & 1) * 32;  // the wave's 32 x 32 quarter
+    const int r = lane & 15, kg = lane >> 4;
+    floatx4 acc[2][2];
 #pragma unroll
-            for (int j = 0; j < 8; j++) {
-                const int k = k0 + 8 * kg + j;
-                vb[j] = (k < g.K && n < g.N) ? g.B[(long long)k * g.sbk + (long long)n * g.sbn] : (bf16)0.0f;
-            }
-            acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(va, vb, acc[t], 0, 0, 0);
-        }
+    for (int i = 0; i < 2; i++)
+#pragma unroll
+        for (int j = 0; j < 2; j++) acc[i][j] = floatx4{0.0f, 0.0f, 0.0f, 0.0f};
+    for (int k0 = 0; k0 < g.K; k0 += kGK) {
+        __syncthreads();
+        gemm_stage(g.A, g.sam, g.sak, g.M, g.K, m0, k0, As, tid);
+        // B(k, n): its "rows" are the n columns, its k stride is sbk
+        gemm_stage(g.B, g.sbn, g.sbk, g.N, g.K, n0, k0, Bs, tid);
+        __syncthreads();
+        bf16x8 a[2], b[2];
+#pragma unroll
+        for (int i = 0; i < 2; i++) a[i] = *reinterpret_cast<const bf16x8 *>(&As[wm + 16 * i + r][8 * kg]);
+#pragma unroll
+        for (int j = 0; j < 2; j++) b[j] = *reinterpret_cast<const bf16x8 *>(&Bs[wn + 16 * j + r][8 * kg]);
+#pragma unroll
+        for (int i = 0; i < 2; i++)
+#pragma unroll
+            for (int j = 0; j < 2; j++) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[i], b[j], acc[i][j], 0, 0, 0);
     }
 #pragma unroll
-    for (int t = 0; t < NT; t++) {
-        const int col = (tg * NT + t) * 16 + r;
-        if (col >= g.N) continue;
+    for (int i = 0; i < 2; i++)
 #pragma unroll
-        for (int j = 0; j < 4; j++) {
-            const int row = tm * 16 + 4 * kg + j;
-            if (row >= g.M) continue;
-            const float v = acc[t][j];
-            if (g.mode == 0) {
-                g.C[(long long)row * g.ldc + col] = v + (g.bias ? g.bias[col] : 0.0f);
-            } else if (g.mode == 1) {
-                const float z = v + g.bias[col];
-                g.Z[(long long)row * g.ldz + col] = z;
-                g.H[(long long)row * g.ldh + col] = (bf16)gelu(z);
-            } else if (g.mode == 2) {
-                g.H[(long long)row * g.ldh + col] = (bf16)(v * gelu_grad(g.Z[(long long)row * g.ldz + col]));
-            } else {
-                g.H[(long long)row * g.ldh + col] = (bf16)gelu(v + g.bias[col]);
+        for (int j = 0; j < 2; j++) {
+            const int col = n0 + wn + 16 * j + r;
+            if (col >= g.N) continue;
+#pragma unroll
+            for (int q = 0; q < 4; q++) {
+                const int row = m0 + wm + 16 * i + 4 * kg + q;
+                if (row >= g.M) continue;
+                const float v = acc[i][j][q];
+                if (g.mode == 0) {
+                    g.C[(long long)row * g.ldc + col] = v + (g.bias ? g.bias[col] : 0.0f);
+                } else if (g.mode == 1) {
+                    const float z = v + g.bias[col];
+                    g.Z[(long long)row * g.ldz + col] = z;
+                    g.H[(long long)row * g.ldh + col] = (bf16)gelu(z);
+                } else if (g.mode == 2) {
+                    g.H[(long long)row * g.ldh + col] = (bf16)(v * gelu_grad(g.Z[(long long)row * g.ldz + col]));
+                } else {
+                    g.H[(long long)row * g.ldh + col] = (bf16)gelu(v + g.bias[col]);
+                }
             }
         }
-    }
 }
 
 // One wave per searched spectrum: the features (the inference kernel's own
@@ -265,15 +292,8 @@ hipError_t launch_gemm(const GemmArgs &g, hipStream_t s) {
     if ((g.mode == 0 && !g.C) || ((g.mode == 1 || g.mode == 3) && (!g.bias || !g.H)) ||
         (g.mode == 1 && !g.Z) || (g.mode == 2 && (!g.Z || !g.H)))
         return hipErrorInvalidValue;
-    const int tiles_n = (g.N + 15) / 16;
-    const long long tiles_m = (g.M + 15) / 16;
-    if (tiles_n >= 16) {
-        const long long strips = tiles_m * ((tiles_n + 15) / 16);
-        hipLaunchKernelGGL(gemm_kernel<16>, dim3((unsigned)((strips + 3) / 4)), dim3(256), 0, s, g);
-    } else {
-        const long long strips = tiles_m * tiles_n;
-        hipLaunchKernelGGL(gemm_kernel<1>, dim3((unsigned)((strips + 3) / 4)), dim3(256), 0, s, g);
-    }
+    const long long tiles = (long long)((g.M + kGT - 1) / kGT) * ((g.N + kGT - 1) / kGT);
+    hipLaunchKernelGGL(gemm_kernel, dim3((unsigned)tiles), dim3(256), 0, s, g);
     return hipGetLastError();
 }
 
