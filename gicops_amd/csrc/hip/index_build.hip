@@ -9,14 +9,15 @@
 //                    per-site additions of UTILS_GenerateModSpectrum,
 //                    utils.cpp:347-474; the precursor-mass filter and clamp
 //                    of dslim.cpp:318-337), keyed by ion id;
-//   radix sort       stable, by bin, so ion ids ascend within each bin (the
-//                    SLM transform, dslim.cpp:376-402, 474-504);
+//   radix sort       rocPRIM's device radix sort of (bin, ion id) pairs over
+//                    the bin's bits: stable, so ion ids ascend within each
+//                    bin (the SLM transform, dslim.cpp:376-402, 474-504);
 //   bin_starts       bA[b] = first position of bin b in the sorted keys.
 //
 // The reference's own GPU build sums in another order (superstep1/kernel.cu
 // :635-733), which moves some ions across bin boundaries; this one is
 // bit-identical to the CPU index.
-#include <hipcub/hipcub.hpp>
+#include <rocprim/device/device_radix_sort.hpp>
 
 #include <vector>
 
@@ -222,11 +223,11 @@ int build_bucket_device(gx_ctx *c, const gx_bucket &x, const gx_index_build_para
     int end_bit = 1;
     while (end_bit < 32 && (1ull << end_bit) < nbins) end_bit++;
     size_t tmp_bytes = 0;
-    if (hipcub::DeviceRadixSort::SortPairs(nullptr, tmp_bytes, keys, keys2, vals, iA, (int)n, 0, end_bit, s) !=
+    if (rocprim::radix_sort_pairs(nullptr, tmp_bytes, keys, keys2, vals, iA, (size_t)n, 0u, (unsigned)end_bit, s) !=
         hipSuccess)
         return fail(GX_ERR_DEVICE);
     if (hipMalloc(&tmp, tmp_bytes ? tmp_bytes : 1) != hipSuccess) return fail(GX_ERR_BAD_MEM_ALLOC);
-    if (hipcub::DeviceRadixSort::SortPairs(tmp, tmp_bytes, keys, keys2, vals, iA, (int)n, 0, end_bit, s) !=
+    if (rocprim::radix_sort_pairs(tmp, tmp_bytes, keys, keys2, vals, iA, (size_t)n, 0u, (unsigned)end_bit, s) !=
         hipSuccess)
         return fail(GX_ERR_DEVICE);
     hipLaunchKernelGGL(bin_starts_kernel, dim3((nbins + 1 + 255) / 256), dim3(256), 0, s, keys2, (uint32_t)n, nbins,

@@ -12,9 +12,9 @@
 //   pick_kernel    one thread per spectrum: the quicksort's own partition
 //                  steps on the spectrum's (intensity, m/z) pairs, then the
 //                  normalisation and cut; writes the kept count;
-//   exclusive scan of the kept counts (hipcub) -> the batch's CSR offsets;
+//   exclusive scan of the kept counts (rocPRIM) -> the batch's CSR offsets;
 //   compact_kernel one wave per spectrum copies its kept tail.
-#include <hipcub/hipcub.hpp>
+#include <rocprim/device/device_scan.hpp>
 
 #include <algorithm>
 #include <vector>
@@ -175,7 +175,8 @@ extern "C" int gx_queries_preprocess(gx_ctx *c, const gx_query_batch *q, const g
         return fail(GX_ERR_BAD_MEM_ALLOC);
     hipStream_t st = c->stream;
     size_t tmp_bytes = 0;
-    if (hipcub::DeviceScan::ExclusiveSum(nullptr, tmp_bytes, kept, d->idx, (int)ns + 1, st) != hipSuccess)
+    if (rocprim::exclusive_scan(nullptr, tmp_bytes, kept, d->idx, 0u, (size_t)ns + 1, rocprim::plus<uint32_t>(), st) !=
+        hipSuccess)
         return fail(GX_ERR_DEVICE);
     if (hipMalloc(&tmp, tmp_bytes ? tmp_bytes : 1) != hipSuccess) return fail(GX_ERR_BAD_MEM_ALLOC);
     if ((q->idx && hipMemcpyAsync(r_idx, q->idx, 4 * ((size_t)ns + 1), hipMemcpyHostToDevice, st) != hipSuccess) ||
@@ -191,7 +192,8 @@ extern "C" int gx_queries_preprocess(gx_ctx *c, const gx_query_batch *q, const g
         hipLaunchKernelGGL(pick_kernel, dim3((ns + 63) / 64), dim3(64), 0, st, pa);
         if (hipGetLastError() != hipSuccess) return fail(GX_ERR_DEVICE);
     }
-    if (hipcub::DeviceScan::ExclusiveSum(tmp, tmp_bytes, kept, d->idx, (int)ns + 1, st) != hipSuccess)
+    if (rocprim::exclusive_scan(tmp, tmp_bytes, kept, d->idx, 0u, (size_t)ns + 1, rocprim::plus<uint32_t>(), st) !=
+        hipSuccess)
         return fail(GX_ERR_DEVICE);
     if (ns > 0) {
         CompactArgs ca{r_idx, kept, d->idx, r_in, r_mz, d->inten, d->moz, ns};
