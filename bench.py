@@ -219,6 +219,41 @@ def calibrate_copy(mib: int) -> int:
     return 0
 
 
+def mlp_gemm(rows: int, calls: int = 5) -> int:
+    """Only the surrogate's forward and backward passes (eight bf16 MFMA GEMMs
+    per call: 128-256-256-16 forward; dW3, dZ2, dW2, dZ1, dW1 backward) on
+    `rows` synthetic rows, for the GEMM kernel's throughput under a kernel
+    trace. FLOPs per row: 2*(128*256 + 256*256 + 256*16) forward, and
+    2*(256*16 + 16*256 + 2*256*256 + 128*256) backward."""
+    ensure_built()
+    from gicops_amd import slm
+    from gicops_amd.search import Searcher
+    from gicops_amd.surrogate import Surrogate
+
+    index = slm.SlmIndex(slm.DB_PRESETS["tiny"])
+    s = Searcher(index)
+    sur = Surrogate(s, seed=3)
+    rng = np.random.default_rng(0)
+    feat = rng.standard_normal((rows, 128)).astype(np.float32)
+    target = np.zeros((rows, 4), np.float32)
+    target[:, 0] = rng.integers(0, 50, rows)
+    target[:, 1] = target[:, 0] + rng.integers(1, 50, rows)
+    target[:, 2] = 100.0
+    target[:, 3] = 1.0
+    norm = np.array([0.2, 0.1, 0.8, 0.1], np.float32)
+    t = time.perf_counter()
+    for _ in range(calls):
+        sur.grad(feat, target, norm, 0.5)
+    wall = time.perf_counter() - t
+    flops = rows * (2 * (128 * 256 + 256 * 256 + 256 * 16) + 2 * (256 * 16 + 16 * 256 + 2 * 256 * 256 + 128 * 256))
+    print(json.dumps({"mlp_gemm_rows": rows, "calls": calls, "gemm_flops_per_call": flops,
+                      "wall_s_with_host_copies": round(wall, 3)}), flush=True)
+    sur.close()
+    s.close()
+    index.close()
+    return 0
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -247,9 +282,13 @@ def main() -> int:
                     help="skip the read-bandwidth probe (for short profiler runs)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run a device-to-device copy of MIB MiB (HBM counter calibration) and exit")
+    ap.add_argument("--mlp-gemm", type=int, default=0, metavar="ROWS",
+                    help="only run 5 forward+backward passes of the surrogate on ROWS synthetic rows and exit")
     args = ap.parse_args()
     if args.calibrate_copy:
         return calibrate_copy(args.calibrate_copy)
+    if args.mlp_gemm:
+        return mlp_gemm(args.mlp_gemm)
     if not args.train_spectra:
         args.train_spectra = 100_000 if args.workload in ("c4", "uneven") else 500_000
     if args.gpus != 1:

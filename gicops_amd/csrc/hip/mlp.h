@@ -74,6 +74,16 @@ struct GemmArgs {
     long long ldz;
     bf16 *H;             // modes 1, 2 output [M][N] (row stride ldh)
     long long ldh;
+    // mode 0 only (optional): workspace for splitting K over blocks when the
+    // output has few tiles (weight gradients, K = rows); the partial tiles are
+    // summed in a fixed order, so results do not depend on scheduling
+    float *ws;
+    long long ws_floats;
+    // mode 0 only (optional): A gets an extra row M of ones, whose output
+    // row, the column sums of B (a bias gradient), goes here [N]
+    float *colsum_out;
+    int ksplit;          // set by launch_gemm
+    int kchunk;          // set by launch_gemm
 };
 
 hipError_t launch_gemm(const GemmArgs &g, hipStream_t s);
@@ -107,7 +117,6 @@ struct LossArgs {
 };
 hipError_t launch_loss(const LossArgs &a, hipStream_t s);
 
-hipError_t launch_colsum(const bf16 *x, uint32_t rows, uint32_t cols, float *out, hipStream_t s);
 hipError_t launch_adam(float *w, const float *g, float *m, float *v, uint32_t n, float lr, float b1, float b2,
                        float eps, float b1t, float b2t, hipStream_t s);
 hipError_t launch_to_bf16(const float *x, bf16 *y, uint64_t n, hipStream_t s);

@@ -1,10 +1,13 @@
 // Learned e-value surrogate kernels for gfx950: a strided BF16 GEMM on
 // v_mfma_f32_16x16x32_bf16 with fused epilogues (bias + GELU forward, GELU'
-// backward) for training, the training rows of searched spectra (features
-// and the plain fit's exact marks, expeRT.cpp:477-515), the loss with the
-// window-check label, bias gradients, Adam, and the weights rearranged as
-// MFMA B fragments for the fused inference kernel (learned.hip).
+// backward; split K and the bias gradients as an extra row of ones for the
+// weight gradients) for training, the training rows of searched spectra
+// (features and the plain fit's exact marks, expeRT.cpp:477-515), the loss
+// with the window-check label, Adam, and the weights rearranged as MFMA B
+// fragments for the fused inference kernel (learned.hip).
 #include <math.h>
+
+#include <algorithm>
 
 #include "learned_util.h"
 
@@ -26,14 +29,18 @@ __device__ __forceinline__ float gelu_grad(float z) { return mlp_gelu_grad(z); }
 // lane l supplies A[row l%16][k0 + 8*(l/16) .. +7] and B[k0 + 8*(l/16) ..
 // +7][col l%16] and receives C[4*(l/16) + j][l%16], j < 4. Each wave computes
 // a 32 x 32 quarter (2 x 2 MFMA tiles). Elements outside M, N, K are zero, so
-// any shape works; the epilogue modes are those of GemmArgs.
+// any shape works; the epilogue modes are those of GemmArgs. With ksplit > 1
+// (mode 0), block b computes tile b % tiles over the k range of chunk
+// b / tiles into its own slice of the workspace, and splitk_reduce_kernel
+// adds the slices in chunk order.
 constexpr int kGT = 64;        // block tile
 constexpr int kGK = 32;        // k step
 constexpr int kGPad = kGK + 8; // LDS row stride (bf16): 80 B spreads a fragment read over the banks
 
 __device__ __forceinline__ void gemm_stage(const bf16 *P, long long s_row, long long s_k, int rows, int K, int r0,
-                                           int k0, bf16 (*T)[kGPad], int tid) {
-    // T[r][k] = P(r0 + r, k0 + k), 64 x 32
+                                           int k0, bf16 (*T)[kGPad], int tid, int ones_row) {
+    // T[r][k] = P(r0 + r, k0 + k), 64 x 32; row ones_row (>= rows, or -1 for
+    // none) is all ones
     if (s_k == 1) {
         // 8 consecutive k of one row per thread
         const int r = tid >> 2, kc = (tid & 3) * 8;
@@ -45,6 +52,10 @@ __device__ __forceinline__ void gemm_stage(const bf16 *P, long long s_row, long 
         } else {
 #pragma unroll
             for (int j = 0; j < 8; j++) v[j] = (gr < rows && gk + j < K) ? src[j] : (bf16)0.0f;
+            if (gr == ones_row) {
+#pragma unroll
+                for (int j = 0; j < 8; j++) v[j] = gk + j < K ? (bf16)1.0f : (bf16)0.0f;
+            }
         }
         *reinterpret_cast<bf16x8 *>(&T[r][kc]) = v;
     } else {
@@ -58,6 +69,9 @@ __device__ __forceinline__ void gemm_stage(const bf16 *P, long long s_row, long 
         } else {
 #pragma unroll
             for (int j = 0; j < 8; j++) v[j] = (gk < K && gr + j < rows) ? src[(long long)j * s_row] : (bf16)0.0f;
+#pragma unroll
+            for (int j = 0; j < 8; j++)
+                if (gr + j == ones_row) v[j] = gk < K ? (bf16)1.0f : (bf16)0.0f;
         }
 #pragma unroll
         for (int j = 0; j < 8; j++) T[rc + j][k] = v[j];
@@ -68,8 +82,14 @@ __global__ void __launch_bounds__(256) gemm_kernel(GemmArgs g) {
     __shared__ __align__(16) bf16 As[kGT][kGPad];
     __shared__ __align__(16) bf16 Bs[kGT][kGPad];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int rows_c = g.M + (g.colsum_out ? 1 : 0);  // rows of the product, with the ones row
     const int tiles_n = (g.N + kGT - 1) / kGT;
-    const int m0 = (blockIdx.x / tiles_n) * kGT, n0 = (blockIdx.x % tiles_n) * kGT;
+    const int tiles = ((rows_c + kGT - 1) / kGT) * tiles_n;
+    const int split = (int)blockIdx.x / tiles, tile = (int)blockIdx.x % tiles;
+    const int m0 = (tile / tiles_n) * kGT, n0 = (tile % tiles_n) * kGT;
+    // k range of this block (the whole of K without a split; chunks are whole k steps)
+    const int kb = g.ksplit > 1 ? split * g.kchunk : 0;
+    const int ke = g.ksplit > 1 ? min(g.K, kb + g.kchunk) : g.K;
     const int wm = (wave >> 1) * 32, wn = (wave & 1) * 32;  // the wave's 32 x 32 quarter
     const int r = lane & 15, kg = lane >> 4;
     floatx4 acc[2][2];
@@ -77,11 +97,11 @@ __global__ void __launch_bounds__(256) gemm_kernel(GemmArgs g) {
     for (int i = 0; i < 2; i++)
 #pragma unroll
         for (int j = 0; j < 2; j++) acc[i][j] = floatx4{0.0f, 0.0f, 0.0f, 0.0f};
-    for (int k0 = 0; k0 < g.K; k0 += kGK) {
+    for (int k0 = kb; k0 < ke; k0 += kGK) {
         __syncthreads();
-        gemm_stage(g.A, g.sam, g.sak, g.M, g.K, m0, k0, As, tid);
+        gemm_stage(g.A, g.sam, g.sak, g.M, g.K, m0, k0, As, tid, g.colsum_out ? g.M : -1);
         // B(k, n): its "rows" are the n columns, its k stride is sbk
-        gemm_stage(g.B, g.sbn, g.sbk, g.N, g.K, n0, k0, Bs, tid);
+        gemm_stage(g.B, g.sbn, g.sbk, g.N, g.K, n0, k0, Bs, tid, -1);
         __syncthreads();
         bf16x8 a[2], b[2];
 #pragma unroll
@@ -102,9 +122,13 @@ __global__ void __launch_bounds__(256) gemm_kernel(GemmArgs g) {
 #pragma unroll
             for (int q = 0; q < 4; q++) {
                 const int row = m0 + wm + 16 * i + 4 * kg + q;
-                if (row >= g.M) continue;
+                if (row >= rows_c) continue;
                 const float v = acc[i][j][q];
-                if (g.mode == 0) {
+                if (g.ksplit > 1) {
+                    g.ws[((long long)split * rows_c + row) * g.N + col] = v;
+                } else if (row == g.M) {
+                    g.colsum_out[col] = v;
+                } else if (g.mode == 0) {
                     g.C[(long long)row * g.ldc + col] = v + (g.bias ? g.bias[col] : 0.0f);
                 } else if (g.mode == 1) {
                     const float z = v + g.bias[col];
@@ -117,6 +141,29 @@ __global__ void __launch_bounds__(256) gemm_kernel(GemmArgs g) {
                 }
             }
         }
+}
+
+// C = the sum of the ksplit partial products (+ bias), in a fixed order: four
+// running sums over chunks s = 4 q + u, then (u0 + u1) + (u2 + u3), so that
+// four loads are in flight per step
+__global__ void __launch_bounds__(256) splitk_reduce_kernel(GemmArgs g) {
+    const long long i = (long long)blockIdx.x * 256 + threadIdx.x;
+    const int rows_c = g.M + (g.colsum_out ? 1 : 0);
+    const long long mn = (long long)rows_c * g.N;
+    if (i >= mn) return;
+    const int row = (int)(i / g.N), col = (int)(i % g.N);
+    float v[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+    int s = 0;
+    for (; s + 4 <= g.ksplit; s += 4) {
+#pragma unroll
+        for (int u = 0; u < 4; u++) v[u] += g.ws[(long long)(s + u) * mn + i];
+    }
+    for (int u = 0; s < g.ksplit; s++, u++) v[u] += g.ws[(long long)s * mn + i];
+    const float t = (v[0] + v[1]) + (v[2] + v[3]);
+    if (row == g.M)
+        g.colsum_out[col] = t;
+    else
+        g.C[(long long)row * g.ldc + col] = t + (g.bias ? g.bias[col] : 0.0f);
 }
 
 // One wave per searched spectrum: the features (the inference kernel's own
@@ -180,11 +227,12 @@ __global__ void __launch_bounds__(64) features_kernel(FeatArgs a) {
 // Mean squared error of the two normalised positions (servable rows) plus
 // conf_weight times the cross-entropy of the confidence against the label
 // "both marks lie in the check windows around the predicted indexes".
-__global__ void loss_kernel(LossArgs a) {
+__global__ void __launch_bounds__(256) loss_kernel(LossArgs a) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     __shared__ float acc[4];
     if (threadIdx.x < 4) acc[threadIdx.x] = 0.0f;
     __syncthreads();
+    float part[4] = {0.0f, 0.0f, 0.0f, 0.0f};
     if (i < a.n) {
         const float *o = a.out + (size_t)i * kOutPad;
         const float *t = a.target + (size_t)i * kTarget;
@@ -202,38 +250,34 @@ __global__ void loss_kernel(LossArgs a) {
         const float p = 1.0f / (1.0f + expf(-o[2]));
         const float bce = -(y * logf(fmaxf(p, 1e-7f)) + (1.0f - y) * logf(fmaxf(1.0f - p, 1e-7f)));
         const float inv = 1.0f / (float)a.n;
-        bf16 *d = a.dout + (size_t)i * kOutPad;
-        d[0] = (bf16)(2.0f * e0 * inv);
-        d[1] = (bf16)(2.0f * e1 * inv);
-        d[2] = (bf16)(a.conf_weight * (p - y) * inv);
-        for (int j = 3; j < kOutPad; j++) d[j] = (bf16)0.0f;
-        atomicAdd(&acc[0], e0 * e0);
-        atomicAdd(&acc[1], e1 * e1);
-        atomicAdd(&acc[2], bce);
-        atomicAdd(&acc[3], y);
+        // the row's 16 output gradients (3 used) as two 16-byte stores
+        static_assert(kOutPad == 16, "two bf16x8 stores per row");
+        bf16x8 d0, d1;
+#pragma unroll
+        for (int j = 0; j < 8; j++) d0[j] = d1[j] = (bf16)0.0f;
+        d0[0] = (bf16)(2.0f * e0 * inv);
+        d0[1] = (bf16)(2.0f * e1 * inv);
+        d0[2] = (bf16)(a.conf_weight * (p - y) * inv);
+        bf16x8 *d = reinterpret_cast<bf16x8 *>(a.dout + (size_t)i * kOutPad);
+        d[0] = d0;
+        d[1] = d1;
+        part[0] = e0 * e0;
+        part[1] = e1 * e1;
+        part[2] = bce;
+        part[3] = y;
+    }
+    // the block's sums: wave sums, then one LDS add per wave
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) part[k] += __shfl_xor(part[k], o, 64);
+    }
+    if ((threadIdx.x & 63) == 0) {
+#pragma unroll
+        for (int k = 0; k < 4; k++) atomicAdd(&acc[k], part[k]);
     }
     __syncthreads();
     if (threadIdx.x < 4) atomicAdd(&a.loss[threadIdx.x], acc[threadIdx.x]);
-}
-
-// column sums of a bf16 [rows][cols] matrix (bias gradients): one block per
-// 64 columns, its 16 waves over interleaved rows (a wave reads 128 B of one
-// row), then a reduction over the waves in LDS
-__global__ void __launch_bounds__(1024) colsum_kernel(const bf16 *x, uint32_t rows, uint32_t cols, float *out) {
-    __shared__ float part[16][64];
-    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    const uint32_t c = blockIdx.x * 64 + lane;
-    float s = 0.0f;
-    if (c < cols)
-        for (uint32_t r = w; r < rows; r += 16) s += (float)x[(size_t)r * cols + c];
-    part[w][lane] = s;
-    __syncthreads();
-    if (w == 0 && c < cols) {
-        float t = 0.0f;
-#pragma unroll
-        for (int k = 0; k < 16; k++) t += part[k][lane];
-        out[c] = t;
-    }
 }
 
 __global__ void adam_kernel(float *w, const float *g, float *m, float *v, uint32_t n, float lr, float b1, float b2,
@@ -288,12 +332,31 @@ __global__ void frag_weights_kernel(const float *w, bf16 *frag) {
 
 hipError_t launch_gemm(const GemmArgs &g, hipStream_t s) {
     if (g.M <= 0 || g.N <= 0) return hipSuccess;
-    if (g.K < 0 || g.mode < 0 || g.mode > 3) return hipErrorInvalidValue;
+    if (g.K < 0 || g.mode < 0 || g.mode > 3 || (g.colsum_out && g.mode != 0)) return hipErrorInvalidValue;
     if ((g.mode == 0 && !g.C) || ((g.mode == 1 || g.mode == 3) && (!g.bias || !g.H)) ||
         (g.mode == 1 && !g.Z) || (g.mode == 2 && (!g.Z || !g.H)))
         return hipErrorInvalidValue;
-    const long long tiles = (long long)((g.M + kGT - 1) / kGT) * ((g.N + kGT - 1) / kGT);
-    hipLaunchKernelGGL(gemm_kernel, dim3((unsigned)tiles), dim3(256), 0, s, g);
+    const long long rows_c = g.M + (g.colsum_out ? 1 : 0);
+    const long long tiles = (long long)((rows_c + kGT - 1) / kGT) * ((g.N + kGT - 1) / kGT);
+    GemmArgs a = g;
+    a.ksplit = 1;
+    a.kchunk = g.K;
+    if (g.mode == 0 && g.ws && tiles < 256) {
+        // about 512 blocks, at least 4 k steps per chunk, within the workspace
+        const long long want = (512 + tiles - 1) / tiles;
+        const long long by_k = g.K / (4 * kGK);
+        const long long by_ws = g.ws_floats / (rows_c * g.N);
+        const long long sp = std::min(want, std::min(by_k, by_ws));
+        if (sp > 1) {
+            a.kchunk = (int)(((g.K + sp - 1) / sp + kGK - 1) / kGK * kGK);
+            a.ksplit = (g.K + a.kchunk - 1) / a.kchunk;
+        }
+    }
+    hipLaunchKernelGGL(gemm_kernel, dim3((unsigned)(tiles * a.ksplit)), dim3(256), 0, s, a);
+    if (a.ksplit > 1) {
+        const long long mn = rows_c * g.N;
+        hipLaunchKernelGGL(splitk_reduce_kernel, dim3((unsigned)((mn + 255) / 256)), dim3(256), 0, s, a);
+    }
     return hipGetLastError();
 }
 
@@ -306,11 +369,6 @@ hipError_t launch_features(const FeatArgs &a, hipStream_t s) {
 hipError_t launch_loss(const LossArgs &a, hipStream_t s) {
     if (a.n == 0) return hipSuccess;
     hipLaunchKernelGGL(loss_kernel, dim3((a.n + 255) / 256), dim3(256), 0, s, a);
-    return hipGetLastError();
-}
-
-hipError_t launch_colsum(const bf16 *x, uint32_t rows, uint32_t cols, float *out, hipStream_t s) {
-    hipLaunchKernelGGL(colsum_kernel, dim3((cols + 63) / 64), dim3(1024), 0, s, x, rows, cols, out);
     return hipGetLastError();
 }
 

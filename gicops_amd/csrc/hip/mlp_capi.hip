@@ -19,6 +19,8 @@ struct Surrogate {
     float *w = nullptr, *g = nullptr, *m = nullptr, *v = nullptr;
     bf16 *wb = nullptr;    // bf16 copy in MlpLayout (training GEMMs)
     bf16 *frag = nullptr;  // bf16 MFMA B fragments (FragLayout, inference)
+    float *ws = nullptr;   // split-K partial products of the weight-gradient GEMMs
+    static constexpr long long kWsFloats = 1ll << 22;
     MlpNorm norm{0.0f, 1.0f, 0.0f, 1.0f};
     bool norm_set = false;
     long long step = 0;
@@ -105,6 +107,8 @@ hipError_t forward(gx_mlp *M, const bf16 *x, uint32_t n, hipStream_t s, bool kee
 hipError_t backward(gx_mlp *M, const bf16 *x, uint32_t n, hipStream_t s) {
     hipError_t e;
     GemmArgs g{};
+    g.ws = M->ws;
+    g.ws_floats = M->ws ? Surrogate::kWsFloats : 0;
     // dW3 = H2^T dO
     g.M = kHidden;
     g.N = kOutPad;
@@ -118,8 +122,8 @@ hipError_t backward(gx_mlp *M, const bf16 *x, uint32_t n, hipStream_t s) {
     g.mode = 0;
     g.C = M->g + MlpLayout::W3;
     g.ldc = kOutPad;
+    g.colsum_out = M->g + MlpLayout::B3;  // db3 = column sums of dO
     if ((e = launch_gemm(g, s))) return e;
-    if ((e = launch_colsum(M->dout, n, kOutPad, M->g + MlpLayout::B3, s))) return e;
     // dZ2 = (dO W3^T) * gelu'(Z2)
     GemmArgs d{};
     d.M = (int)n;
@@ -149,8 +153,8 @@ hipError_t backward(gx_mlp *M, const bf16 *x, uint32_t n, hipStream_t s) {
     g.sbn = 1;
     g.C = M->g + MlpLayout::W2;
     g.ldc = kHidden;
+    g.colsum_out = M->g + MlpLayout::B2;  // db2 = column sums of dZ2
     if ((e = launch_gemm(g, s))) return e;
-    if ((e = launch_colsum(M->dz2, n, kHidden, M->g + MlpLayout::B2, s))) return e;
     // dZ1 = (dZ2 W2^T) * gelu'(Z1)
     d.K = kHidden;
     d.A = M->dz2;
@@ -173,8 +177,8 @@ hipError_t backward(gx_mlp *M, const bf16 *x, uint32_t n, hipStream_t s) {
     g.sbn = 1;
     g.C = M->g + MlpLayout::W1;
     g.ldc = kHidden;
-    if ((e = launch_gemm(g, s))) return e;
-    return launch_colsum(M->dz1, n, kHidden, M->g + MlpLayout::B1, s);
+    g.colsum_out = M->g + MlpLayout::B1;  // db1 = column sums of dZ1
+    return launch_gemm(g, s);
 }
 
 }  // namespace
@@ -191,8 +195,8 @@ extern "C" int gx_mlp_create(gx_ctx *c, gx_mlp **out) {
     M->ctx = c;
     if (dalloc(&M->w, MlpLayout::N) || dalloc(&M->g, MlpLayout::N) || dalloc(&M->m, MlpLayout::N) ||
         dalloc(&M->v, MlpLayout::N) || dalloc(&M->wb, MlpLayout::N) || dalloc(&M->frag, FragLayout::N) ||
-        dalloc(&M->loss, 4)) {
-        free_all({M->w, M->g, M->m, M->v, M->wb, M->frag, M->loss});
+        dalloc(&M->loss, 4) || dalloc(&M->ws, (size_t)Surrogate::kWsFloats)) {
+        free_all({M->w, M->g, M->m, M->v, M->wb, M->frag, M->loss, M->ws});
         delete M;
         return GX_ERR_BAD_MEM_ALLOC;
     }
@@ -208,7 +212,7 @@ extern "C" void gx_mlp_destroy(gx_mlp *M) {
         M->ctx->surrogate = nullptr;
         M->ctx->scratch_gen++;
     }
-    free_all({M->w, M->g, M->m, M->v, M->wb, M->frag, M->loss, M->ds_feat, M->ds_tgt, M->ds_loge, M->x, M->h1, M->h2, M->dz1,
+    free_all({M->w, M->g, M->m, M->v, M->wb, M->frag, M->loss, M->ws, M->ds_feat, M->ds_tgt, M->ds_loge, M->x, M->h1, M->h2, M->dz1,
               M->dz2, M->dout, M->tg, M->lg, M->z1, M->z2, M->out, M->idx});
     delete M;
 }

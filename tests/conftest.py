@@ -4,6 +4,13 @@ from pathlib import Path
 
 import pytest
 
+try:
+    # torch's HIP runtime has to open the device before the first test loads
+    # libgicops_hip.so, whichever test files are selected and in whatever order
+    import torch  # noqa: F401
+except ImportError:
+    pass
+
 ROOT = Path(__file__).resolve().parent.parent
 if str(ROOT) not in sys.path:
     sys.path.insert(0, str(ROOT))

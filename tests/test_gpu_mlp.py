@@ -46,7 +46,8 @@ def test_mfma_forward_matches_oracle(searcher_open, n):
     s.close()
 
 
-@pytest.mark.parametrize("n", [50, 64])
+# 1500 rows: the weight-gradient GEMMs (K = rows) split K over blocks
+@pytest.mark.parametrize("n", [50, 64, 1500])
 def test_gradients_match_oracle(searcher_open, n):
     say(f"case: MLP gradient n={n}")
     w = _params(7 + n)
