@@ -428,7 +428,7 @@ __global__ void __launch_bounds__(kSliceThreads) plan_slices_kernel(PlanArgs a) 
 template <int THREADS, int TILE_WORDS, int SL>
 struct SpecLds {
     alignas(16) uint32_t cnt[TILE_WORDS];  // count tile; pass-2 accumulators and the peak index after pass 1
-    uint32_t s_a[SL];          // non-empty slices of the tile: first ion
+    uint32_t s_a[SL];          // non-empty slices of the tile: first ion minus the slice's flat offset
     uint32_t s_flat[SL + 1];   // exclusive prefix of their lengths
     uint32_t wsum[THREADS / 64];
     uint32_t wsum2[THREADS / 64];
@@ -798,7 +798,7 @@ __global__ void __launch_bounds__(THREADS, WAVES_PER_SIMD) search_spectra_kernel
                     // starting before each chunk. The masks are clear on entry.
                     if (fl) {
                         const uint32_t f = bl + il - len;
-                        L.s_a[bf + ifl - 1] = at;
+                        L.s_a[bf + ifl - 1] = at - f;  // ion position = s_a + flat index (mod 2^32)
                         L.s_flat[bf + ifl - 1] = f;
                         if ((f >> 6) < (uint32_t)THREADS) atomicOr(&L.cmask[f >> 6], 1ull << (f & 63u));
                     }
@@ -844,7 +844,7 @@ __global__ void __launch_bounds__(THREADS, WAVES_PER_SIMD) search_spectra_kernel
                         const unsigned long long upto =
                             lane == 63 ? ~0ull : ((1ull << (uint32_t)(lane + 1)) - 1ull);
                         for (uint32_t c = (uint32_t)wave * U; c < nr; c += NW * U) {
-                            uint32_t j0[U], sa[U], sf[U], raw[U];
+                            uint32_t j0[U], sa[U], raw[U];
                             unsigned long long mk[U];
 #pragma unroll
                             for (int u = 0; u < U; u++) {
@@ -857,14 +857,13 @@ __global__ void __launch_bounds__(THREADS, WAVES_PER_SIMD) search_spectra_kernel
                                 // slices starting at or before this lane, minus one
                                 const uint32_t j = j0[u] + (uint32_t)__popcll(mk[u] & upto) - 1u;
                                 sa[u] = L.s_a[j];
-                                sf[u] = L.s_flat[j];
                             }
 #pragma unroll
                             for (int u = 0; u < U; u++) {
                                 const uint32_t f = ((r0 + c + (uint32_t)u) << 6) + (uint32_t)lane;
                                 const bool ok = c + (uint32_t)u < nr && f < N;
                                 // a failed lane gets an id just below the tile: filtered below
-                                raw[u] = ok ? ia[sa[u] + (f - sf[u])] : tlo_id - 1u;
+                                raw[u] = ok ? ia[sa[u] + f] : tlo_id - 1u;
                             }
 #pragma unroll
                             for (int u = 0; u < U; u++) {
