@@ -846,11 +846,23 @@ __global__ void __launch_bounds__(THREADS, WAVES_PER_SIMD) search_spectra_kernel
                         for (uint32_t c = (uint32_t)wave * U; c < nr; c += NW * U) {
                             uint32_t j0[U], sa[U], raw[U];
                             unsigned long long mk[U];
+                            {
+                                // lane u reads chunk c + u's metadata (uniform per chunk); the
+                                // wave takes it from that lane
+                                uint32_t my_j0 = 0;
+                                unsigned long long my_mk = 0;
+                                if (lane < U) {
+                                    const uint32_t cc = min(c + (uint32_t)lane, nr - 1);
+                                    my_j0 = L.cj0[cc];
+                                    my_mk = L.cmask[cc];
+                                }
 #pragma unroll
-                            for (int u = 0; u < U; u++) {
-                                const uint32_t cc = min(c + (uint32_t)u, nr - 1);
-                                j0[u] = L.cj0[cc];
-                                mk[u] = L.cmask[cc];
+                                for (int u = 0; u < U; u++) {
+                                    j0[u] = (uint32_t)__builtin_amdgcn_readlane((int)my_j0, u);
+                                    const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)my_mk, u);
+                                    const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(my_mk >> 32), u);
+                                    mk[u] = ((unsigned long long)hi << 32) | lo;
+                                }
                             }
 #pragma unroll
                             for (int u = 0; u < U; u++) {
