@@ -874,8 +874,11 @@ __global__ void __launch_bounds__(THREADS, WAVES_PER_SIMD) search_spectra_kernel
                             for (int u = 0; u < U; u++) {
                                 const uint32_t f = ((r0 + c + (uint32_t)u) << 6) + (uint32_t)lane;
                                 const bool ok = c + (uint32_t)u < nr && f < N;
-                                // a failed lane gets an id just below the tile: filtered below
-                                raw[u] = ok ? ia[sa[u] + f] : tlo_id - 1u;
+                                // every lane loads (a lane past the round's ions reads entry
+                                // 0, a valid address), so no branch splits the ten loads; a
+                                // failed lane gets an id just below the tile: filtered below
+                                const uint32_t v = ia[ok ? sa[u] + f : 0u];
+                                raw[u] = ok ? v : tlo_id - 1u;
                             }
 #pragma unroll
                             for (int u = 0; u < U; u++) {
