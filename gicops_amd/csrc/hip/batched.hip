@@ -31,7 +31,7 @@ constexpr int kPlanThreads = 256;
 constexpr int kSliceThreads = 128;  // plan_slices block (small blocks keep more of them in flight)
 constexpr int kPeakCap = kMaxPeaksLds;  // 2048 legal peaks per spectrum
 constexpr int kStatSpread = kStatRows;
-constexpr int kUnroll = 5;              // stream chunks in flight per wave: 2 * kUnroll
+constexpr int kUnroll = 6;              // stream chunks in flight per wave: 2 * kUnroll
 constexpr int kCandLdsSpec = 640;       // candidates kept in LDS per spectrum (the rest in global scratch)
 constexpr uint32_t kPsidMask = (1u << 26) - 1u;
 constexpr uint32_t kWholeFlag = 0x80000000u;  // marks a whole-bin slice start (ion positions < 2^31)
