@@ -430,8 +430,8 @@ struct SpecLds {
     alignas(16) uint32_t cnt[TILE_WORDS];  // count tile; pass-2 accumulators and the peak index after pass 1
     uint32_t s_a[SL];          // non-empty slices of the tile: first ion minus the slice's flat offset
     uint32_t s_flat[SL + 1];   // exclusive prefix of their lengths
-    uint32_t wsum[THREADS / 64];
-    uint32_t wsum2[THREADS / 64];
+    alignas(16) uint32_t wsum[THREADS / 64];   // per-wave partial sums (read back as 16-byte vectors)
+    alignas(16) uint32_t wsum2[THREADS / 64];
     uint16_t cj0[THREADS];     // slices starting before each chunk of the round
     unsigned long long cmask[THREADS];  // lanes of each chunk where a slice starts
     uint32_t ne;
@@ -787,9 +787,15 @@ __global__ void __launch_bounds__(THREADS, WAVES_PER_SIMD) search_spectra_kernel
                     }
                     lds_barrier();
                     uint32_t bl = 0, bf = 0;
-                    for (int w = 0; w < wave; w++) {
-                        bl += L.wsum[w];
-                        bf += L.wsum2[w];
+                    static_assert(NW % 4 == 0, "per-wave sums read four at a time");
+#pragma unroll
+                    for (int g = 0; g < NW / 4; g++) {
+                        const uint4 x = reinterpret_cast<const uint4 *>(L.wsum)[g];
+                        const uint4 y = reinterpret_cast<const uint4 *>(L.wsum2)[g];
+                        bl += (4 * g + 0 < wave ? x.x : 0u) + (4 * g + 1 < wave ? x.y : 0u) +
+                              (4 * g + 2 < wave ? x.z : 0u) + (4 * g + 3 < wave ? x.w : 0u);
+                        bf += (4 * g + 0 < wave ? y.x : 0u) + (4 * g + 1 < wave ? y.y : 0u) +
+                              (4 * g + 2 < wave ? y.z : 0u) + (4 * g + 3 < wave ? y.w : 0u);
                     }
                     // the tile's ions, flattened over the non-empty slices, in rounds
                     // of THREADS 64-id chunks. Each slice sets the bit of its first
@@ -828,10 +834,12 @@ __global__ void __launch_bounds__(THREADS, WAVES_PER_SIMD) search_spectra_kernel
                             if (lane == 63) L.wsum[wave] = inc;
                             lds_barrier();
                             uint32_t wb = 0, tot = 0;
-                            for (int w = 0; w < NW; w++) {
-                                const uint32_t x = L.wsum[w];
-                                wb += w < wave ? x : 0u;
-                                tot += x;
+#pragma unroll
+                            for (int g = 0; g < NW / 4; g++) {
+                                const uint4 x = reinterpret_cast<const uint4 *>(L.wsum)[g];
+                                wb += (4 * g + 0 < wave ? x.x : 0u) + (4 * g + 1 < wave ? x.y : 0u) +
+                                      (4 * g + 2 < wave ? x.z : 0u) + (4 * g + 3 < wave ? x.w : 0u);
+                                tot += (x.x + x.y) + (x.z + x.w);
                             }
                             if ((uint32_t)tid < nr) L.cj0[tid] = (uint16_t)(base + wb + inc - v);
                             base += tot;
