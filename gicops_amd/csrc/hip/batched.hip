@@ -75,7 +75,7 @@ __global__ void __launch_bounds__(kPlanThreads) plan_spectra_kernel(PlanArgs a) 
         const uint64_t m = __ballot(legal);
         uint32_t wbase = 0;
         if (lane == 0 && m) wbase = atomicAdd(&s_nlegal, (uint32_t)__popcll(m));
-        wbase = __shfl(wbase, 0, 64);
+        wbase = (uint32_t)__builtin_amdgcn_readlane((int)wbase, 0);
         if (legal) {
             const uint32_t pos = wbase + (uint32_t)__popcll(m & ((1ull << lane) - 1));
             tq[pos] = q;
@@ -388,7 +388,7 @@ __global__ void __launch_bounds__(kSliceThreads) plan_slices_kernel(PlanArgs a) 
         const unsigned long long km = __ballot(keep);
         uint32_t wb0 = 0;
         if (lane == 0 && km) wb0 = (uint32_t)atomicAdd(&s_st[0], (unsigned long long)__popcll(km));
-        wb0 = (uint32_t)__shfl(wb0, 0, 64);
+        wb0 = (uint32_t)__builtin_amdgcn_readlane((int)wb0, 0);
         if (!keep) continue;
         const uint32_t jc = wb0 + (uint32_t)__popcll(km & below);
         if (whole) {

@@ -142,6 +142,14 @@ __device__ __forceinline__ unsigned long long wave_max_u64(unsigned long long v)
 }
 
 // inclusive prefix sum across a wave
+// lane l's double in every lane (two v_readlane, no LDS)
+__device__ __forceinline__ double readlane_f64(double v, int l) {
+    const long long b = __double_as_longlong(v);
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)b, l);
+    const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)((unsigned long long)b >> 32), l);
+    return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+
 // inclusive wave scan on DPP moves (no LDS traffic): shifts within each row of
 // 16 lanes, then row 15's and row 31's last lanes broadcast into the rows
 // above them. Call with every lane of the wave active.
@@ -313,7 +321,7 @@ __device__ bool tail_fit_wave(const uint32_t *hist, double *sx, int hyp, int cps
             if (x > 1.0) x = 0.999;
             sx[i] = x;
         }
-        carry = __shfl(inc, 63, 64);
+        carry = (uint32_t)__builtin_amdgcn_readlane((int)inc, 63);
     }
     int r = -1;
     for (int i = lane; i < n; i += 64)

@@ -64,7 +64,7 @@ __device__ bool snap_fit(const uint32_t *cpre, int n, int stt1, uint32_t cps, in
     // the thresholds from the curve's ends (expeRT.cpp:477-485)
     double e = 0.0;
     if (lane < 2) e = log_survival(cpre[lane == 0 ? 0 : n - 1], vaa);
-    const double s0 = __shfl(e, 0, 64), sN = __shfl(e, 1, 64);
+    const double s0 = readlane_f64(e, 0), sN = readlane_f64(e, 1);
     const double hgt = sN - s0;
     const double t1 = s0 + hgt * 0.22, t2 = s0 + hgt * 0.87;
     const bool lo_half = lane < kSnapLanes;

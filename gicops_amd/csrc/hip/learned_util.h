@@ -74,7 +74,7 @@ __device__ inline void curve_prefix(const uint32_t *h, const Curve &c, int lane,
         const int i = k * 64 + lane;
         const uint32_t inc = wave_incl_scan(v[k], lane) + carry;
         if (i < c.n) cpre[i] = inc;
-        carry = __shfl(inc, 63, 64);
+        carry = (uint32_t)__builtin_amdgcn_readlane((int)inc, 63);
     }
 }
 
@@ -103,7 +103,7 @@ __device__ inline Curve curve_load(const uint32_t *h, int hyp, int lane, uint32_
         }
         const uint32_t inc = wave_incl_scan(v[k], lane) + carry;
         if (b < top) pre[b] = inc;
-        carry = __shfl(inc, 63, 64);
+        carry = (uint32_t)__builtin_amdgcn_readlane((int)inc, 63);
     }
     e = wave_max_i32(e);
     st = wave_min_i32(st);
