@@ -819,7 +819,7 @@ __global__ void __launch_bounds__(THREADS, WAVES_PER_SIMD) search_spectra_kernel
                     uint32_t base = 0;  // slices starting before the round
                     for (uint32_t r0 = 0; r0 < nch; r0 += THREADS) {
                         const uint32_t nr = min((uint32_t)THREADS, nch - r0);
-                        if (r0 > 0) {  // later rounds: the masks were cleared after the previous one
+                        if (r0 > 0) {  // later rounds: the previous round's readers cleared the masks
                             lds_barrier();
                             if ((uint32_t)tid < ne) {
                                 const uint32_t f = L.s_flat[tid];
@@ -855,14 +855,18 @@ __global__ void __launch_bounds__(THREADS, WAVES_PER_SIMD) search_spectra_kernel
                             uint32_t j0[U], sa[U], raw[U];
                             unsigned long long mk[U];
                             {
-                                // lane u reads chunk c + u's metadata (uniform per chunk); the
-                                // wave takes it from that lane
+                                // lane u reads chunk c + u's metadata (uniform per chunk) and,
+                                // as the chunk's only reader, clears its mask for the next
+                                // round or step (their mask writes come after at least one
+                                // barrier); the wave takes the metadata from that lane. A chunk
+                                // past the round is a dummy whose lanes are all filtered.
                                 uint32_t my_j0 = 0;
-                                unsigned long long my_mk = 0;
-                                if (lane < U) {
-                                    const uint32_t cc = min(c + (uint32_t)lane, nr - 1);
+                                unsigned long long my_mk = 1;
+                                if (lane < U && c + (uint32_t)lane < nr) {
+                                    const uint32_t cc = c + (uint32_t)lane;
                                     my_j0 = L.cj0[cc];
                                     my_mk = L.cmask[cc];
+                                    L.cmask[cc] = 0;
                                 }
 #pragma unroll
                                 for (int u = 0; u < U; u++) {
@@ -913,8 +917,6 @@ __global__ void __launch_bounds__(THREADS, WAVES_PER_SIMD) search_spectra_kernel
                                 }
                             }
                         }
-                        lds_barrier();
-                        L.cmask[tid] = 0;  // ordered before the next use by at least one barrier
                     }
                 }
                 lds_barrier();
