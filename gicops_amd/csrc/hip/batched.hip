@@ -424,6 +424,15 @@ __global__ void __launch_bounds__(kSliceThreads) plan_slices_kernel(PlanArgs a) 
     }
 }
 
+// a slice starting at flat index f marks its 64-id chunk's mask: lane k >= 1 of
+// the chunk sets bit k - 1 and lane 0 sets bit 63, so that a lane's count of
+// slice starts at or before it is mbcnt over the mask plus bit 63, and the
+// mask's popcount is still the chunk's number of slice starts
+__device__ __forceinline__ unsigned long long chunk_bit(uint32_t f) {
+    const uint32_t k = f & 63u;
+    return k ? 1ull << (k - 1u) : 1ull << 63;
+}
+
 // ---------------------------------------------------------------------------
 template <int THREADS, int TILE_WORDS, int SL>
 struct SpecLds {
@@ -806,7 +815,7 @@ __global__ void __launch_bounds__(THREADS, WAVES_PER_SIMD) search_spectra_kernel
                         const uint32_t f = bl + il - len;
                         L.s_a[bf + ifl - 1] = at - f;  // ion position = s_a + flat index (mod 2^32)
                         L.s_flat[bf + ifl - 1] = f;
-                        if ((f >> 6) < (uint32_t)THREADS) atomicOr(&L.cmask[f >> 6], 1ull << (f & 63u));
+                        if ((f >> 6) < (uint32_t)THREADS) atomicOr(&L.cmask[f >> 6], chunk_bit(f));
                     }
                     if (tid == THREADS - 1) {
                         L.s_flat[bf + ifl] = bl + il;
@@ -824,7 +833,7 @@ __global__ void __launch_bounds__(THREADS, WAVES_PER_SIMD) search_spectra_kernel
                             if ((uint32_t)tid < ne) {
                                 const uint32_t f = L.s_flat[tid];
                                 const uint32_t c = (f >> 6) - r0;
-                                if (f >= (r0 << 6) && c < nr) atomicOr(&L.cmask[c], 1ull << (f & 63u));
+                                if (f >= (r0 << 6) && c < nr) atomicOr(&L.cmask[c], chunk_bit(f));
                             }
                             lds_barrier();
                         }
@@ -849,8 +858,6 @@ __global__ void __launch_bounds__(THREADS, WAVES_PER_SIMD) search_spectra_kernel
                         // metadata, slice bases, ion ids; lane L's slice is the chunk's
                         // first plus the slice starts at lanes <= L
                         constexpr int U = 2 * kUnroll;
-                        const unsigned long long upto =
-                            lane == 63 ? ~0ull : ((1ull << (uint32_t)(lane + 1)) - 1ull);
                         for (uint32_t c = (uint32_t)wave * U; c < nr; c += NW * U) {
                             uint32_t j0[U], sa[U], raw[U];
                             unsigned long long mk[U];
@@ -861,7 +868,7 @@ __global__ void __launch_bounds__(THREADS, WAVES_PER_SIMD) search_spectra_kernel
                                 // barrier); the wave takes the metadata from that lane. A chunk
                                 // past the round is a dummy whose lanes are all filtered.
                                 uint32_t my_j0 = 0;
-                                unsigned long long my_mk = 1;
+                                unsigned long long my_mk = 1ull << 63;
                                 if (lane < U && c + (uint32_t)lane < nr) {
                                     const uint32_t cc = c + (uint32_t)lane;
                                     my_j0 = L.cj0[cc];
@@ -878,8 +885,12 @@ __global__ void __launch_bounds__(THREADS, WAVES_PER_SIMD) search_spectra_kernel
                             }
 #pragma unroll
                             for (int u = 0; u < U; u++) {
-                                // slices starting at or before this lane, minus one
-                                const uint32_t j = j0[u] + (uint32_t)__popcll(mk[u] & upto) - 1u;
+                                // slices starting at or before this lane, minus one: a slice
+                                // starting at lane 0 (bit 63) counts for every lane, one at
+                                // lane k >= 1 (bit k - 1) for lanes >= k, i.e. bits below the lane
+                                const uint32_t jb = j0[u] + (uint32_t)(mk[u] >> 63) - 1u;
+                                const uint32_t j = __builtin_amdgcn_mbcnt_hi(
+                                    (uint32_t)(mk[u] >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)mk[u], jb));
                                 sa[u] = L.s_a[j];
                             }
 #pragma unroll
@@ -897,11 +908,13 @@ __global__ void __launch_bounds__(THREADS, WAVES_PER_SIMD) search_spectra_kernel
                                 // ids of the tile's entries are [tlo_id, tlo_id + span_id);
                                 // with the exact division (checked on the host for every
                                 // id < 2^31) that gives 0 <= el < nent
-                                const bool in_tile = raw[u] - tlo_id < span_id;
+                                const uint32_t dr = raw[u] - tlo_id;
+                                const bool in_tile = dr < span_id;
                                 // the wave's count of consumed ids (uniform: kept in a scalar register)
                                 n_scanned += (uint32_t)__popcll(__ballot(in_tile));
                                 if (in_tile) {
-                                    const uint32_t el = (__umulhi(raw[u], dm) >> dsh) - (uint32_t)tlo;
+                                    // floor(raw / speclen) - tlo, as tlo_id = tlo * speclen
+                                    const uint32_t el = __umulhi(dr, dm) >> dsh;
                                     const uint32_t sh = (el & (pack - 1)) << fsh;
                                     const uint32_t old = atomicAdd(&L.cnt[el >> lp], 1u << sh);
                                     // the increment that reaches min_shp makes the entry a
