@@ -735,6 +735,7 @@ __global__ void __launch_bounds__(THREADS, WAVES_PER_SIMD) search_spectra_kernel
             const DevBucket bk = a.buckets[b];
             const uint32_t speclen = bk.speclen;
             const auto ia = gptr(bk.iA);
+            const uint32_t sent = bk.n_entries * speclen;  // iA's sentinel slot
             const uint32_t dm = bk.div_m, dsh = bk.div_s;
             const uint32_t btag = b << 26;
             for (uint32_t t = 0; t < tiles; t++) {
@@ -897,11 +898,10 @@ __global__ void __launch_bounds__(THREADS, WAVES_PER_SIMD) search_spectra_kernel
                             for (int u = 0; u < U; u++) {
                                 const uint32_t f = ((r0 + c + (uint32_t)u) << 6) + (uint32_t)lane;
                                 const bool ok = c + (uint32_t)u < nr && f < N;
-                                // every lane loads (a lane past the round's ions reads entry
-                                // 0, a valid address), so no branch splits the ten loads; a
-                                // failed lane gets an id just below the tile: filtered below
-                                const uint32_t v = ia[ok ? sa[u] + f : 0u];
-                                raw[u] = ok ? v : tlo_id - 1u;
+                                // every lane loads, so no branch splits the loads: a lane past
+                                // the round's ions reads the bucket's sentinel (no ion id),
+                                // which the tile filter below drops
+                                raw[u] = ia[ok ? sa[u] + f : sent];
                             }
 #pragma unroll
                             for (int u = 0; u < U; u++) {

@@ -220,7 +220,7 @@ int index_create(gx_ctx *c, const gx_bucket *b, int nb, int maxz, uint32_t *cons
             uint32_t *bA = on_dev ? dev_bA[i] : nullptr, *iA = on_dev ? dev_iA[i] : nullptr, *ib = nullptr,
                      *is = nullptr;
             if (on_dev && (!bA || !iA)) return fail(GX_ERR_INVLD_PTR);
-            if (dalloc(&m, x.n_entries) || (!on_dev && (dalloc(&bA, (size_t)x.nbins + 1) || dalloc(&iA, x.n_ions))) ||
+            if (dalloc(&m, x.n_entries) || (!on_dev && (dalloc(&bA, (size_t)x.nbins + 1) || dalloc(&iA, (size_t)x.n_ions + 1))) ||
                 dalloc(&ib, x.n_ions) || dalloc(&is, (x.n_ions + 31) / 32)) {
                 for (void *q : {(void *)m, (void *)ib, (void *)is}) if (q) (void)hipFree(q);
                 if (!on_dev) {
@@ -243,6 +243,11 @@ int index_create(gx_ctx *c, const gx_bucket *b, int nb, int maxz, uint32_t *cons
                 return fail(GX_ERR_DEVICE);
             if (launch_ion_bin(bA, iA, ib, x.nbins, x.n_ions, c->stream) != hipSuccess ||
                 launch_sample32(iA, is, x.n_ions, c->stream) != hipSuccess)
+                return fail(GX_ERR_DEVICE);
+            // iA[n_ions] holds no ion id (0xffffffff): the search kernel's lanes past a
+            // round's ions read it and drop it with the ids outside their tile
+            const uint32_t sentinel = 0xffffffffu;
+            if (hipMemcpy(iA + x.n_ions, &sentinel, sizeof(sentinel), hipMemcpyHostToDevice) != hipSuccess)
                 return fail(GX_ERR_DEVICE);
             db.mass = m;
             db.bA = bA;

@@ -14,7 +14,7 @@ namespace gx {
 struct DevBucket {
     const float *mass;
     const uint32_t *bA;
-    const uint32_t *iA;
+    const uint32_t *iA;     // n_entries * speclen ion ids, then one 0xffffffff sentinel
     const uint32_t *ion_bin;
     const uint32_t *iS;     // every 32nd ion id: iS[k] = iA[32k] (boundary searches)
     uint32_t n_entries;
