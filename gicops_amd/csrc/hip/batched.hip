@@ -561,7 +561,9 @@ __device__ void rescore_candidates(const ItemArgs &a, SpecLds<THREADS, TILE_WORD
                     n_err += ok && (cd >> 26) != b ? 1u : 0u;
                     cl[u] = ok ? ci - c0 : 0xffffffffu;
                     rr[u] = r;
-                    x[u] = ok ? (int32_t)ib[(size_t)(cd & kPsidMask) * spl + r] : 0;
+                    // every task loads (a task past the run reads bin 0 of the bucket,
+                    // unused: cl marks it), so no select waits for the load
+                    x[u] = (int32_t)ib[ok ? (size_t)(cd & kPsidMask) * spl + r : 0];
                 }
 #pragma unroll
                 for (int u = 0; u < U; u++) {
