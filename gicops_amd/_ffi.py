@@ -82,6 +82,7 @@ class gx_search_params(C.Structure):
         ("min_cpsm", C.c_int32), ("evalue_mode", C.c_int32), ("dM", C.c_double),
         ("expect_max", C.c_double), ("surrogate_conf", C.c_float), ("engine", C.c_int32),
         ("config", C.c_int32), ("use_graph", C.c_int32), ("split", C.c_int32), ("whole_bin", C.c_int32),
+        ("wide_stream", C.c_int32),
     ]
 
 

@@ -86,6 +86,7 @@ struct ItemArgs {
     uint32_t *cand_scratch;  // per block overflow candidates
     int tile_entries_max;
     int split;               // window split: psids per item = tile / split
+    int wide;                // 1: ion stream by 64-bit addresses (the WIDE kernel instance)
     uint32_t max_speclen;    // pass-2 task stride, with its exact divisor
     uint32_t smax_m, smax_s;
     uint32_t cell_shift;     // bins per coarse peak-index cell = 2^cell_shift
@@ -132,6 +133,9 @@ hipError_t launch_plan_spectra(const PlanArgs &a, hipStream_t s);
 hipError_t launch_scan(const PlanArgs &a, hipStream_t s);
 hipError_t launch_plan_slices(const PlanArgs &a, hipStream_t s);
 hipError_t launch_search_spectra(int config, const ItemArgs &a, int n_blocks, hipStream_t s);
+// whether the search kernel's 32-bit ion-stream offsets reach every ion of a
+// bucket of n_ions ions (and its padding)
+bool narrow_stream_fits(uint64_t n_ions);
 hipError_t launch_evalue(const EvalArgs &a, hipStream_t s);
 hipError_t launch_fold(const FoldArgs &a, hipStream_t s);
 

@@ -32,6 +32,16 @@ constexpr int kSliceThreads = 128;  // plan_slices block (small blocks keep more
 constexpr int kPeakCap = kMaxPeaksLds;  // 2048 legal peaks per spectrum
 constexpr int kStatSpread = kStatRows;
 constexpr int kUnroll = 6;              // stream chunks in flight per wave: 2 * kUnroll
+// The ion stream addresses iA as a 32-bit byte offset from a scalar base plus
+// the instruction's immediate offset 256 u (chunk u of a batch). The offsets
+// are kept kStreamBias bytes high, so that offset - 256 u never goes below
+// zero. An index with a bucket whose iA (with its padding) is too large for
+// that is searched by the kernel's WIDE instance: ion positions, 64-bit
+// addresses.
+constexpr uint32_t kStreamBias = 256u * 2u * kUnroll;
+constexpr uint64_t kNarrowIons = ((1ull << 32) - kStreamBias) / 4u - kIaPad;
+// lanes past a round read at most 64 (2 kUnroll) - 1 ids into the padding
+static_assert(64u * 2u * kUnroll <= kIaPad, "iA's padding covers a batch of dummy chunks");
 constexpr int kCandLdsSpec = 640;       // candidates kept in LDS per spectrum (the rest in global scratch)
 constexpr uint32_t kPsidMask = (1u << 26) - 1u;
 constexpr uint32_t kWholeFlag = 0x80000000u;  // marks a whole-bin slice start (ion positions < 2^31)
@@ -437,8 +447,13 @@ __device__ __forceinline__ unsigned long long chunk_bit(uint32_t f) {
 template <int THREADS, int TILE_WORDS, int SL>
 struct SpecLds {
     alignas(16) uint32_t cnt[TILE_WORDS];  // count tile; pass-2 accumulators and the peak index after pass 1
-    uint32_t s_a[SL];          // non-empty slices of the tile: first ion minus the slice's flat offset
+    // non-empty slices of the tile: first ion minus the slice's flat offset, as
+    // a biased byte offset (an ion position in the WIDE instance); entry ne is
+    // the virtual slice past the last one and ne + 1 the round's dummy slot,
+    // both of which lead into iA's padding
+    uint32_t s_a[SL + 2];
     uint32_t s_flat[SL + 1];   // exclusive prefix of their lengths
+    uint32_t dummy[64];        // per-lane word that out-of-tile lanes add 0 to
     alignas(16) uint32_t wsum[THREADS / 64];   // per-wave partial sums (read back as 16-byte vectors)
     alignas(16) uint32_t wsum2[THREADS / 64];
     uint16_t cj0[THREADS];     // slices starting before each chunk of the round
@@ -640,7 +655,7 @@ __device__ void rescore_candidates(const ItemArgs &a, SpecLds<THREADS, TILE_WORD
 // tile), then collects the tile's entries with count >= min_shp as
 // candidates. Pass 2 rescans all candidates of the spectrum at once; the
 // histogram, candidate count and top hit are written once per spectrum.
-template <int THREADS, int TILE_WORDS, int SL, int WAVES_PER_SIMD>
+template <int THREADS, int TILE_WORDS, int SL, int WAVES_PER_SIMD, bool WIDE>
 __global__ void __launch_bounds__(THREADS, WAVES_PER_SIMD) search_spectra_kernel(ItemArgs a) {
     constexpr int NW = THREADS / 64;
     __shared__ SpecLds<THREADS, TILE_WORDS, SL> L;
@@ -654,6 +669,9 @@ __global__ void __launch_bounds__(THREADS, WAVES_PER_SIMD) search_spectra_kernel
     for (int w = tid; w < TILE_WORDS; w += THREADS) L.cnt[w] = 0;
     for (int w = tid; w < GX_HIST_BINS; w += THREADS) L.hist[w] = 0;
     L.cmask[tid] = 0;
+    // never changed (out-of-tile lanes add 0): ~(min_cnt - 1), whose low field,
+    // the one such a lane's check reads (its el is 0), never equals min_cnt - 1
+    if (tid < 64) L.dummy[tid] = ~(min_cnt - 1u);
     for (int b = tid; b < a.n_buckets; b += THREADS) L.b_speclen[b] = a.buckets[b].speclen;
     const uint32_t nunits = a.n_units;
     if (tid == 0) {
@@ -731,13 +749,17 @@ __global__ void __launch_bounds__(THREADS, WAVES_PER_SIMD) search_spectra_kernel
             const uint32_t pack = (uint32_t)__builtin_amdgcn_readlane((int)w_pack, (int)b);
             const uint32_t lp = pack == 4 ? 2u : (pack == 2 ? 1u : 0u);  // log2 entries per word
             const uint32_t fbits = 32u >> lp;                             // counter width
-            const uint32_t fmask = fbits == 32 ? 0xffffffffu : ((1u << fbits) - 1u);
+            const uint32_t fbw = fbits == 32 ? 31u : fbits;               // field width for ubfe (counts < 2^31)
             const uint32_t fsh = 5u - lp;                                 // log2 of the counter width
             const uint32_t T = ((uint32_t)TILE_WORDS / (uint32_t)a.split) << lp;
             const DevBucket bk = a.buckets[b];
             const uint32_t speclen = bk.speclen;
             const auto ia = gptr(bk.iA);
-            const uint32_t sent = bk.n_entries * speclen;  // iA's sentinel slot
+            const uint32_t sent = bk.n_entries * speclen;  // iA's padding starts here
+            // slice bases as biased byte offsets (loads from a scalar base, a 32-bit
+            // offset and the immediate), or as ion positions in the WIDE instance
+            const uint32_t so_shift = WIDE ? 0u : 2u, so_bias = WIDE ? 0u : kStreamBias;
+            const auto iab = (const __attribute__((address_space(1))) char *)ia - kStreamBias;
             const uint32_t dm = bk.div_m, dsh = bk.div_s;
             const uint32_t btag = b << 26;
             for (uint32_t t = 0; t < tiles; t++) {
@@ -816,13 +838,20 @@ __global__ void __launch_bounds__(THREADS, WAVES_PER_SIMD) search_spectra_kernel
                     // starting before each chunk. The masks are clear on entry.
                     if (fl) {
                         const uint32_t f = bl + il - len;
-                        L.s_a[bf + ifl - 1] = at - f;  // ion position = s_a + flat index (mod 2^32)
+                        // ion position = s_a + flat index (mod 2^32), as bytes + bias or not
+                        L.s_a[bf + ifl - 1] = ((at - f) << so_shift) + so_bias;
                         L.s_flat[bf + ifl - 1] = f;
                         if ((f >> 6) < (uint32_t)THREADS) atomicOr(&L.cmask[f >> 6], chunk_bit(f));
                     }
                     if (tid == THREADS - 1) {
-                        L.s_flat[bf + ifl] = bl + il;
+                        // past the last slice: a virtual slice starting at N whose
+                        // lanes read iA's padding (its chunk bit is set when N is not
+                        // a multiple of 64, i.e. when a real chunk holds it)
+                        const uint32_t n_fl = bl + il;
+                        L.s_flat[bf + ifl] = n_fl;
                         L.ne = bf + ifl;
+                        L.s_a[bf + ifl] = ((sent - n_fl) << so_shift) + so_bias;
+                        if ((n_fl & 63u) && (n_fl >> 6) < (uint32_t)THREADS) atomicOr(&L.cmask[n_fl >> 6], chunk_bit(n_fl));
                     }
                     lds_barrier();
                     const uint32_t ne = L.ne;
@@ -837,6 +866,10 @@ __global__ void __launch_bounds__(THREADS, WAVES_PER_SIMD) search_spectra_kernel
                                 const uint32_t f = L.s_flat[tid];
                                 const uint32_t c = (f >> 6) - r0;
                                 if (f >= (r0 << 6) && c < nr) atomicOr(&L.cmask[c], chunk_bit(f));
+                            }
+                            if (tid == THREADS - 1 && (N & 63u)) {  // the virtual slice
+                                const uint32_t c = (N >> 6) - r0;
+                                if (N >= (r0 << 6) && c < nr) atomicOr(&L.cmask[c], chunk_bit(N));
                             }
                             lds_barrier();
                         }
@@ -854,6 +887,8 @@ __global__ void __launch_bounds__(THREADS, WAVES_PER_SIMD) search_spectra_kernel
                                 tot += (x.x + x.y) + (x.z + x.w);
                             }
                             if ((uint32_t)tid < nr) L.cj0[tid] = (uint16_t)(base + wb + inc - v);
+                            // the dummy slot: chunks past the round read iA's padding
+                            if (tid == 0) L.s_a[ne + 1] = ((sent - ((r0 + nr) << 6)) << so_shift) + so_bias;
                             base += tot;
                         }
                         lds_barrier();
@@ -869,8 +904,8 @@ __global__ void __launch_bounds__(THREADS, WAVES_PER_SIMD) search_spectra_kernel
                                 // as the chunk's only reader, clears its mask for the next
                                 // round or step (their mask writes come after at least one
                                 // barrier); the wave takes the metadata from that lane. A chunk
-                                // past the round is a dummy whose lanes are all filtered.
-                                uint32_t my_j0 = 0;
+                                // past the round is a dummy whose lanes all read the padding.
+                                uint32_t my_j0 = ne + 1;
                                 unsigned long long my_mk = 1ull << 63;
                                 if (lane < U && c + (uint32_t)lane < nr) {
                                     const uint32_t cc = c + (uint32_t)lane;
@@ -896,15 +931,23 @@ __global__ void __launch_bounds__(THREADS, WAVES_PER_SIMD) search_spectra_kernel
                                     (uint32_t)(mk[u] >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)mk[u], jb));
                                 sa[u] = L.s_a[j];
                             }
+                            // every lane loads, with no select: a lane past the round's
+                            // ions reads the padding (no ion id), dropped with the ids
+                            // outside the tile below
+                            if constexpr (!WIDE) {
+                                const uint32_t f4 = (((r0 + c) << 6) + (uint32_t)lane) << 2;
 #pragma unroll
-                            for (int u = 0; u < U; u++) {
-                                const uint32_t f = ((r0 + c + (uint32_t)u) << 6) + (uint32_t)lane;
-                                const bool ok = c + (uint32_t)u < nr && f < N;
-                                // every lane loads, so no branch splits the loads: a lane past
-                                // the round's ions reads the bucket's sentinel (no ion id),
-                                // which the tile filter below drops
-                                raw[u] = ia[ok ? sa[u] + f : sent];
+                                for (int u = 0; u < U; u++)
+                                    raw[u] = *(gmem<uint32_t> *)(iab + (size_t)(uint32_t)(sa[u] + f4) + 256u * (uint32_t)u);
+                            } else {
+                                const uint32_t f0 = ((r0 + c) << 6) + (uint32_t)lane;
+#pragma unroll
+                                for (int u = 0; u < U; u++) raw[u] = ia[sa[u] + f0 + 64u * (uint32_t)u];
                             }
+                            // the batch's counter increments are issued together, without
+                            // a branch: a lane outside the tile adds 0 to its dummy word.
+                            // One wait then covers all their returned old values.
+                            uint32_t old[U], elv[U];
 #pragma unroll
                             for (int u = 0; u < U; u++) {
                                 // ids of the tile's entries are [tlo_id, tlo_id + span_id);
@@ -914,21 +957,26 @@ __global__ void __launch_bounds__(THREADS, WAVES_PER_SIMD) search_spectra_kernel
                                 const bool in_tile = dr < span_id;
                                 // the wave's count of consumed ids (uniform: kept in a scalar register)
                                 n_scanned += (uint32_t)__popcll(__ballot(in_tile));
-                                if (in_tile) {
-                                    // floor(raw / speclen) - tlo, as tlo_id = tlo * speclen
-                                    const uint32_t el = __umulhi(dr, dm) >> dsh;
-                                    const uint32_t sh = (el & (pack - 1)) << fsh;
-                                    const uint32_t old = atomicAdd(&L.cnt[el >> lp], 1u << sh);
-                                    // the increment that reaches min_shp makes the entry a
-                                    // candidate, exactly once
-                                    if (((old >> sh) & fmask) == min_cnt - 1u) {
-                                        const uint32_t p = atomicAdd(&L.n_cand, 1u);
-                                        const uint32_t cd = btag | ((uint32_t)tlo + el);
-                                        if (p < (uint32_t)kCandLdsSpec)
-                                            L.cand[p] = cd;
-                                        else
-                                            g_cand[p - kCandLdsSpec] = cd;
-                                    }
+                                // floor(raw / speclen) - tlo, as tlo_id = tlo * speclen
+                                const uint32_t el = __umulhi(dr, dm) >> dsh;
+                                const uint32_t sh = (el & (pack - 1)) << fsh;
+                                uint32_t *w = in_tile ? &L.cnt[el >> lp] : &L.dummy[lane];
+                                old[u] = atomicAdd(w, in_tile ? 1u << sh : 0u);
+                                elv[u] = in_tile ? el : 0u;
+                            }
+#pragma unroll
+                            for (int u = 0; u < U; u++) {
+                                // the increment that reaches min_shp makes the entry a
+                                // candidate, exactly once
+                                const uint32_t el = elv[u];
+                                const uint32_t sh = (el & (pack - 1)) << fsh;
+                                if (__builtin_amdgcn_ubfe(old[u], sh, fbw) == min_cnt - 1u) {
+                                    const uint32_t p = atomicAdd(&L.n_cand, 1u);
+                                    const uint32_t cd = btag | ((uint32_t)tlo + el);
+                                    if (p < (uint32_t)kCandLdsSpec)
+                                        L.cand[p] = cd;
+                                    else
+                                        g_cand[p - kCandLdsSpec] = cd;
                                 }
                             }
                         }
@@ -1045,8 +1093,12 @@ __global__ void __launch_bounds__(64) evalue_kernel(EvalArgs a) {
 // it caps the VGPRs so that blocks_per_cu blocks fit
 template <int THREADS, int TILE_WORDS, int SL, int WAVES_PER_SIMD>
 hipError_t launch_spectra(const ItemArgs &a, int n_blocks, hipStream_t s) {
-    hipLaunchKernelGGL((search_spectra_kernel<THREADS, TILE_WORDS, SL, WAVES_PER_SIMD>), dim3(n_blocks), dim3(THREADS),
-                       0, s, a);
+    if (a.wide)
+        hipLaunchKernelGGL((search_spectra_kernel<THREADS, TILE_WORDS, SL, WAVES_PER_SIMD, true>), dim3(n_blocks),
+                           dim3(THREADS), 0, s, a);
+    else
+        hipLaunchKernelGGL((search_spectra_kernel<THREADS, TILE_WORDS, SL, WAVES_PER_SIMD, false>), dim3(n_blocks),
+                           dim3(THREADS), 0, s, a);
     return hipGetLastError();
 }
 
@@ -1088,6 +1140,8 @@ hipError_t launch_plan_slices(const PlanArgs &a, hipStream_t s) {
     hipLaunchKernelGGL(plan_slices_kernel, dim3((unsigned)nblk), dim3(kSliceThreads), 0, s, a);
     return hipGetLastError();
 }
+
+bool narrow_stream_fits(uint64_t n_ions) { return n_ions < kNarrowIons; }
 
 hipError_t launch_search_spectra(int config, const ItemArgs &a, int n_blocks, hipStream_t s) {
     switch (config) {

@@ -88,6 +88,7 @@ int check_params(const gx_search_params *p, const gx_dindex *idx) {
     if (p->evalue_mode != GX_EVALUE_TAILFIT && p->evalue_mode != GX_EVALUE_LEARNED)
         return param_error(GX_ERR_INVLD_PARAM, "unknown evalue_mode");
     if (p->whole_bin < 0 || p->whole_bin > 4096) return param_error(GX_ERR_INVLD_PARAM, "whole_bin outside 0..4096");
+    if (p->wide_stream != 0 && p->wide_stream != 1) return param_error(GX_ERR_INVLD_PARAM, "wide_stream not 0 or 1");
     return GX_SUCCESS;
 }
 
@@ -220,7 +221,7 @@ int index_create(gx_ctx *c, const gx_bucket *b, int nb, int maxz, uint32_t *cons
             uint32_t *bA = on_dev ? dev_bA[i] : nullptr, *iA = on_dev ? dev_iA[i] : nullptr, *ib = nullptr,
                      *is = nullptr;
             if (on_dev && (!bA || !iA)) return fail(GX_ERR_INVLD_PTR);
-            if (dalloc(&m, x.n_entries) || (!on_dev && (dalloc(&bA, (size_t)x.nbins + 1) || dalloc(&iA, (size_t)x.n_ions + 1))) ||
+            if (dalloc(&m, x.n_entries) || (!on_dev && (dalloc(&bA, (size_t)x.nbins + 1) || dalloc(&iA, (size_t)x.n_ions + kIaPad))) ||
                 dalloc(&ib, x.n_ions) || dalloc(&is, (x.n_ions + 31) / 32)) {
                 for (void *q : {(void *)m, (void *)ib, (void *)is}) if (q) (void)hipFree(q);
                 if (!on_dev) {
@@ -244,10 +245,10 @@ int index_create(gx_ctx *c, const gx_bucket *b, int nb, int maxz, uint32_t *cons
             if (launch_ion_bin(bA, iA, ib, x.nbins, x.n_ions, c->stream) != hipSuccess ||
                 launch_sample32(iA, is, x.n_ions, c->stream) != hipSuccess)
                 return fail(GX_ERR_DEVICE);
-            // iA[n_ions] holds no ion id (0xffffffff): the search kernel's lanes past a
-            // round's ions read it and drop it with the ids outside their tile
-            const uint32_t sentinel = 0xffffffffu;
-            if (hipMemcpy(iA + x.n_ions, &sentinel, sizeof(sentinel), hipMemcpyHostToDevice) != hipSuccess)
+            // iA[n_ions .. n_ions + kIaPad) hold no ion id (0xffffffff): the search
+            // kernel's lanes past a round's ions read them and drop them with the ids
+            // outside their tile
+            if (hipMemsetAsync(iA + x.n_ions, 0xff, sizeof(uint32_t) * kIaPad, c->stream) != hipSuccess)
                 return fail(GX_ERR_DEVICE);
             db.mass = m;
             db.bA = bA;
@@ -255,6 +256,7 @@ int index_create(gx_ctx *c, const gx_bucket *b, int nb, int maxz, uint32_t *cons
             db.ion_bin = ib;
             db.iS = is;
             d->max_speclen = std::max(d->max_speclen, x.speclen);
+            d->max_ions = std::max(d->max_ions, (uint64_t)x.n_ions);
         }
         d->h_buckets[i] = db;
     }
@@ -556,6 +558,7 @@ void fill_item_args(Batched &B, gx_ctx *c, const gx_dindex *idx, gx_dqueries *q,
     ia.tile_entries_max = B.cfg.tile_words * 4;
     ia.split = B.split;
     ia.max_speclen = idx->max_speclen;
+    ia.wide = (p->wide_stream || !narrow_stream_fits(idx->max_ions)) ? 1 : 0;
     ia.smax_m = idx->smax_m;
     ia.smax_s = idx->smax_s;
     ia.cell_shift = idx->cell_shift;

@@ -188,7 +188,7 @@ int build_bucket_device(gx_ctx *c, const gx_bucket &x, const gx_index_build_para
     const size_t nseq = (size_t)x.n_peps * (size_t)x.peplen;
     if (dalloc(&d_seqs, nseq) || dalloc(&d_seqid, x.n_entries) || dalloc(&d_modnum, x.n_entries) ||
         dalloc(&d_sites, x.n_entries) || dalloc(&keys, n) || dalloc(&vals, n) || dalloc(&keys2, n) ||
-        dalloc(&iA, n + 1) || dalloc(&bA, (size_t)nbins + 1))  // iA[n]: sentinel, set by index_create
+        dalloc(&iA, n + kIaPad) || dalloc(&bA, (size_t)nbins + 1))  // iA's padding: set by index_create
         return fail(GX_ERR_BAD_MEM_ALLOC);
     if (hipMemcpy(d_seqs, x.seqs, nseq, hipMemcpyHostToDevice) != hipSuccess ||
         hipMemcpy(d_seqid, x.seqid, 4 * (size_t)x.n_entries, hipMemcpyHostToDevice) != hipSuccess ||

@@ -14,7 +14,7 @@ namespace gx {
 struct DevBucket {
     const float *mass;
     const uint32_t *bA;
-    const uint32_t *iA;     // n_entries * speclen ion ids, then one 0xffffffff sentinel
+    const uint32_t *iA;     // n_entries * speclen ion ids, then kIaPad sentinels (0xffffffff)
     const uint32_t *ion_bin;
     const uint32_t *iS;     // every 32nd ion id: iS[k] = iA[32k] (boundary searches)
     uint32_t n_entries;
@@ -59,6 +59,9 @@ struct SearchArgs {
 
 // Limits of the LDS-resident per-spectrum state.
 constexpr int kMaxPeaksLds = 2048;   // legal peaks per spectrum
+// iA is followed by kIaPad sentinel slots (0xffffffff, no ion id): the batched
+// search's lanes past a round's ions read them instead of branching
+constexpr uint32_t kIaPad = 1024;
 constexpr int kMaxBuckets = 64;      // length buckets
 constexpr int kSearchThreads = 512;  // 8 waves of 64
 constexpr int kTileWords = 11264;    // count tile: 2 entries (u16) per word -> 22528 entries

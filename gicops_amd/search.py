@@ -38,6 +38,7 @@ class SearchParams:
     use_graph: bool = False
     split: int = 0
     whole_bin: int = 0
+    wide_stream: int = 0
 
     def to_ffi(self) -> F.gx_search_params:
         p = F.gx_search_params()
@@ -45,7 +46,7 @@ class SearchParams:
         p.min_shp, p.min_cpsm, p.evalue_mode = self.min_shp, self.min_cpsm, self.evalue_mode
         p.dM, p.expect_max, p.surrogate_conf = self.dM, self.expect_max, self.surrogate_conf
         p.engine, p.config, p.use_graph = self.engine, self.config, int(self.use_graph)
-        p.split, p.whole_bin = self.split, self.whole_bin
+        p.split, p.whole_bin, p.wide_stream = self.split, self.whole_bin, self.wide_stream
         return p
 
 

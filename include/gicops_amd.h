@@ -114,6 +114,9 @@ typedef struct gx_search_params {
     int32_t split;         /* batched: window split, psids per item = tile / split (1, 2 or 4; 0 = tuned) */
     int32_t whole_bin;     /* batched: bins of at most this many ions are streamed whole and filtered by
                               id instead of searched for tile boundaries (1..4096; 0 = tuned) */
+    int32_t wide_stream;   /* batched: 1 = address iA with 64-bit ion positions even when 32-bit byte
+                              offsets reach every bucket (what an index with a bucket of about 2^30
+                              ions or more always uses); 0 = the faster 32-bit form where it fits */
 } gx_search_params;
 
 /* ---- per-spectrum result (hCell + Results + e-value) ----------------- */

@@ -44,10 +44,14 @@ def compare(gpu, cpu, hg=None, hc=None, label=""):
 
 
 # every search engine and batched launch configuration
+# (wide_stream: the kernel instance with 64-bit ion-stream addresses, which an
+# index with a bucket of about 2^30 ions or more takes)
 ENGINES = [dict(engine=F.GX_ENGINE_BATCHED, config=0), dict(engine=F.GX_ENGINE_BATCHED, config=1),
            dict(engine=F.GX_ENGINE_BATCHED, config=2), dict(engine=F.GX_ENGINE_BATCHED, config=3),
-           dict(engine=F.GX_ENGINE_SPECTRUM)]
-ENGINE_IDS = ["batched-c0", "batched-c1", "batched-c2", "batched-c3", "spectrum"]
+           dict(engine=F.GX_ENGINE_BATCHED, config=0, wide_stream=1),
+           dict(engine=F.GX_ENGINE_BATCHED, config=1, wide_stream=1), dict(engine=F.GX_ENGINE_SPECTRUM)]
+ENGINE_IDS = ["batched-c0", "batched-c1", "batched-c2", "batched-c3", "batched-c0-wide", "batched-c1-wide",
+              "spectrum"]
 
 
 def check_stats(st, cnt, n):
