@@ -9,9 +9,9 @@
 //                  LDS counters) and the number of psid tiles per window;
 //   scan           item and slice-position offsets per spectrum;
 //   plan_slices    one block per (bucket, spectrum): every (peak, bin) slice
-//                  of iA is cut to the window (its ends bracketed from the
-//                  32x ion-id samples, interior tile boundaries searched
-//                  exactly; small bins are flagged to be streamed whole), and
+//                  of iA is cut to the window and to its tiles (every such
+//                  boundary bracketed within 16 ids from the 16x ion-id
+//                  samples; small bins are flagged to be streamed whole), and
 //                  only slices with ions left get slice positions;
 //   search_spectra persistent blocks pull spectra from a counter; each
 //                  (bucket, tile) item of the spectrum streams its slices'
@@ -211,94 +211,19 @@ __global__ void __launch_bounds__(1024) scan_kernel(PlanArgs a) {
     }
 }
 
-// first position p in [st, en) of a bin's ion ids with iA[p] >= x (en if
-// none): a search over the bin's samples iS (every 32nd id, a 32x smaller
-// array that stays in L2) brackets p within 32 ids, then a search of those
-__device__ __forceinline__ uint32_t lower_bound_sampled(const DevBucket &bk, uint32_t st, uint32_t en, uint32_t x) {
-    uint32_t lo = st, hi = en;
-    const uint32_t k0 = (st + 31u) >> 5;
-    if (en > 0 && k0 <= ((en - 1u) >> 5)) {
-        const uint32_t k1 = (en - 1u) >> 5;
-        const uint32_t kk = k0 + lower_bound_u32(gptr(bk.iS) + k0, k1 - k0 + 1u, x);
-        if (kk <= k1) hi = min(hi, kk << 5);                  // iA[32 kk] >= x
-        if (kk > k0) lo = max(lo, ((kk - 1u) << 5) + 1u);    // iA[32 (kk - 1)] < x
-    }
-    return lo + lower_bound_u32(gptr(bk.iA) + lo, hi - lo, x);
-}
-
-// the same for two keys of one bin at once: the two searches' loads are
-// independent, so each step waits for one latency instead of two
-__device__ __forceinline__ void lower_bound_sampled2(const DevBucket &bk, uint32_t st, uint32_t en, uint32_t x0,
-                                                     uint32_t x1, uint32_t &r0, uint32_t &r1) {
-    uint32_t lo0 = st, hi0 = en, lo1 = st, hi1 = en;
-    const uint32_t k0 = (st + 31u) >> 5;
-    if (en > 0 && k0 <= ((en - 1u) >> 5)) {
-        const uint32_t k1 = (en - 1u) >> 5;
-        const auto is = gptr(bk.iS);
-        uint32_t b0 = k0, b1 = k0, n0 = k1 - k0 + 1u, n1 = n0;
-        while (n0 > 0 || n1 > 0) {
-            const uint32_t h0 = n0 >> 1, h1 = n1 >> 1;
-            const uint32_t v0 = n0 > 0 ? is[b0 + h0] : 0u, v1 = n1 > 0 ? is[b1 + h1] : 0u;
-            if (n0 > 0) {
-                if (v0 < x0) {
-                    b0 += h0 + 1u;
-                    n0 -= h0 + 1u;
-                } else {
-                    n0 = h0;
-                }
-            }
-            if (n1 > 0) {
-                if (v1 < x1) {
-                    b1 += h1 + 1u;
-                    n1 -= h1 + 1u;
-                } else {
-                    n1 = h1;
-                }
-            }
-        }
-        if (b0 <= k1) hi0 = min(hi0, b0 << 5);
-        if (b0 > k0) lo0 = max(lo0, ((b0 - 1u) << 5) + 1u);
-        if (b1 <= k1) hi1 = min(hi1, b1 << 5);
-        if (b1 > k0) lo1 = max(lo1, ((b1 - 1u) << 5) + 1u);
-    }
-    const auto ia = gptr(bk.iA);
-    uint32_t n0 = hi0 - lo0, n1 = hi1 - lo1;
-    while (n0 > 0 || n1 > 0) {
-        const uint32_t h0 = n0 >> 1, h1 = n1 >> 1;
-        const uint32_t v0 = n0 > 0 ? ia[lo0 + h0] : 0u, v1 = n1 > 0 ? ia[lo1 + h1] : 0u;
-        if (n0 > 0) {
-            if (v0 < x0) {
-                lo0 += h0 + 1u;
-                n0 -= h0 + 1u;
-            } else {
-                n0 = h0;
-            }
-        }
-        if (n1 > 0) {
-            if (v1 < x1) {
-                lo1 += h1 + 1u;
-                n1 -= h1 + 1u;
-            } else {
-                n1 = h1;
-            }
-        }
-    }
-    r0 = lo0;
-    r1 = lo1;
-}
-
 // brackets of the first positions in [st, en) with iA[p] >= x0 and >= x1 from
-// the bin's samples alone: p0 in [lo0, hi0], p1 in [lo1, hi1], each bracket
-// within one 32-id block (one 128 B line of iA)
+// the bin's samples iS (every kSampleStep-th id) alone: p0 in [lo0, hi0], p1 in
+// [lo1, hi1], each with hi - lo < kSampleStep. The two searches' loads are
+// independent, so each step waits for one latency instead of two.
 __device__ __forceinline__ void bracket_sampled2(const DevBucket &bk, uint32_t st, uint32_t en, uint32_t x0, uint32_t x1,
                                                  uint32_t &lo0, uint32_t &hi0, uint32_t &lo1, uint32_t &hi1) {
     lo0 = st;
     hi0 = en;
     lo1 = st;
     hi1 = en;
-    const uint32_t k0 = (st + 31u) >> 5;
-    if (en > 0 && k0 <= ((en - 1u) >> 5)) {
-        const uint32_t k1 = (en - 1u) >> 5;
+    const uint32_t k0 = (st + kSampleStep - 1u) >> kSampleShift;
+    if (en > 0 && k0 <= ((en - 1u) >> kSampleShift)) {
+        const uint32_t k1 = (en - 1u) >> kSampleShift;
         const auto is = gptr(bk.iS);
         uint32_t b0 = k0, b1 = k0, n0 = k1 - k0 + 1u, n1 = n0;
         while (n0 > 0 || n1 > 0) {
@@ -321,10 +246,10 @@ __device__ __forceinline__ void bracket_sampled2(const DevBucket &bk, uint32_t s
                 }
             }
         }
-        if (b0 <= k1) hi0 = min(hi0, b0 << 5);
-        if (b0 > k0) lo0 = max(lo0, ((b0 - 1u) << 5) + 1u);
-        if (b1 <= k1) hi1 = min(hi1, b1 << 5);
-        if (b1 > k0) lo1 = max(lo1, ((b1 - 1u) << 5) + 1u);
+        if (b0 <= k1) hi0 = min(hi0, b0 << kSampleShift);
+        if (b0 > k0) lo0 = max(lo0, ((b0 - 1u) << kSampleShift) + 1u);
+        if (b1 <= k1) hi1 = min(hi1, b1 << kSampleShift);
+        if (b1 > k0) lo1 = max(lo1, ((b1 - 1u) << kSampleShift) + 1u);
     }
 }
 
@@ -359,9 +284,12 @@ __global__ void __launch_bounds__(kSliceThreads) plan_slices_kernel(PlanArgs a) 
     const int32_t *lq = a.lq + a.qidx[s];
     unsigned long long bins = 0;
     // tile-major rows: row t holds boundary t of every slice, so that the
-    // search kernel's item for tile t reads rows t and t+1 coalesced. Only
-    // non-empty slices get a column (in any order: the search does not depend
-    // on it); their number goes to pair_nsl.
+    // search kernel's item for tile t reads rows t and t+1 coalesced. Row 0 and
+    // row `tiles` bracket the window's ends (the first ion at or below it, the
+    // end at or above it); an interior row t is the lower end of the bracket
+    // of the first ion with id >= (lo + t*T)*speclen, which lies less than
+    // kSampleStep ions past it. Only non-empty slices get a column (in any
+    // order: the search does not depend on it); their number goes to pair_nsl.
     uint32_t *out = a.pos + base;
     const unsigned long long below = lane ? ((1ull << (uint32_t)lane) - 1ull) : 0ull;
     for (int j0 = 0; j0 < nsl; j0 += kSliceThreads) {
@@ -376,17 +304,13 @@ __global__ void __launch_bounds__(kSliceThreads) plan_slices_kernel(PlanArgs a) 
             en = gptr(bk.bA)[bin + 1];
             bins++;
             // bins of at most whole_bin ions are streamed whole by the search
-            // kernel (every tile filters by id). Larger ones get rows 0 and
-            // `tiles` from the window's ends (below) and, in between, row t =
-            // the first ion of the slice with id >= (lo + t*T)*speclen.
+            // kernel (every tile filters by id). Larger ones get rows bracketed
+            // from the samples: the search kernel reads fewer than kSampleStep
+            // ids past each end of a tile's part of the slice and drops the ids
+            // outside the tile. A window from the bucket's first entry starts at
+            // the bin's start, one to its last entry ends at the bin's end.
             whole = en - st <= a.whole_bin;
             if (!whole) {
-                // the window's ends are only bracketed, from the samples: the
-                // slice starts at most 31 ids early and ends at most 31 ids late,
-                // within the 128 B lines its exact ends touch, and the search
-                // kernel drops ids outside the tile. A window from the bucket's
-                // first entry starts at the bin's start, one to its last entry
-                // ends at the bin's end.
                 uint32_t l0, h0, l1, h1;
                 bracket_sampled2(bk, st, en, lo * speclen, end_id, l0, h0, l1, h1);
                 c0 = from_start ? st : l0;
@@ -408,19 +332,27 @@ __global__ void __launch_bounds__(kSliceThreads) plan_slices_kernel(PlanArgs a) 
         }
         out[jc] = c0;
         out[(uint64_t)tiles * nsl + jc] = cz;
-        // interior boundaries two at a time, each pair searched from the previous one
+        // interior boundaries two at a time, each pair bracketed from the
+        // previous one (boundaries ascend with t, and so do their brackets)
         uint32_t cur = c0;
         for (uint32_t t = 1; t < tiles; t += 2) {
             const uint32_t id0 = (lo + t * T) * speclen;
             if (t + 1 < tiles) {
                 const uint32_t id1 = (lo + (t + 1) * T) * speclen;
                 uint32_t b0 = cur, b1 = cur;
-                if (cz > cur) lower_bound_sampled2(bk, cur, cz, id0, id1, b0, b1);
+                if (cz > cur) {
+                    uint32_t h0, h1;
+                    bracket_sampled2(bk, cur, cz, id0, id1, b0, h0, b1, h1);
+                }
                 out[(uint64_t)t * nsl + jc] = b0;
                 out[(uint64_t)(t + 1) * nsl + jc] = b1;
                 cur = b1;
             } else {
-                if (cz > cur) cur = lower_bound_sampled(bk, cur, cz, id0);
+                if (cz > cur) {
+                    uint32_t l0, h0, l1, h1;
+                    bracket_sampled2(bk, cur, cz, id0, id0, l0, h0, l1, h1);
+                    cur = l0;
+                }
                 out[(uint64_t)t * nsl + jc] = cur;
             }
         }
@@ -804,13 +736,15 @@ __global__ void __launch_bounds__(THREADS, WAVES_PER_SIMD) search_spectra_kernel
                     }
                     if (tid < ns) {
                         // a small bin is [start | flag, end in the last row), a larger
-                        // one has tile boundaries in rows t and t+1
+                        // one has the lower ends of its tile boundaries' brackets in rows t
+                        // and t+1: the tile's part ends before row t+1 + kSampleStep (the
+                        // window's end in the last row)
                         if (pt & kWholeFlag) {  // small bin: the whole bin, filtered by id below
                             at = pt & ~kWholeFlag;
                             len = pe - at;
                         } else {
                             at = pt;
-                            len = pt1 - at;
+                            len = min(pt1 + kSampleStep, pe) - at;
                         }
                     }
                     const uint32_t fl = len ? 1u : 0u;

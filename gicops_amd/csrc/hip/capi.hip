@@ -222,7 +222,7 @@ int index_create(gx_ctx *c, const gx_bucket *b, int nb, int maxz, uint32_t *cons
                      *is = nullptr;
             if (on_dev && (!bA || !iA)) return fail(GX_ERR_INVLD_PTR);
             if (dalloc(&m, x.n_entries) || (!on_dev && (dalloc(&bA, (size_t)x.nbins + 1) || dalloc(&iA, (size_t)x.n_ions + kIaPad))) ||
-                dalloc(&ib, x.n_ions) || dalloc(&is, (x.n_ions + 31) / 32)) {
+                dalloc(&ib, x.n_ions) || dalloc(&is, (x.n_ions + kSampleStep - 1) >> kSampleShift)) {
                 for (void *q : {(void *)m, (void *)ib, (void *)is}) if (q) (void)hipFree(q);
                 if (!on_dev) {
                     if (bA) (void)hipFree(bA);
@@ -243,7 +243,7 @@ int index_create(gx_ctx *c, const gx_bucket *b, int nb, int maxz, uint32_t *cons
                   hipMemcpy(iA, x.iA, sizeof(uint32_t) * x.n_ions, hipMemcpyHostToDevice) != hipSuccess)))
                 return fail(GX_ERR_DEVICE);
             if (launch_ion_bin(bA, iA, ib, x.nbins, x.n_ions, c->stream) != hipSuccess ||
-                launch_sample32(iA, is, x.n_ions, c->stream) != hipSuccess)
+                launch_sample_ids(iA, is, x.n_ions, c->stream) != hipSuccess)
                 return fail(GX_ERR_DEVICE);
             // iA[n_ions .. n_ions + kIaPad) hold no ion id (0xffffffff): the search
             // kernel's lanes past a round's ions read them and drop them with the ids

@@ -61,10 +61,11 @@ __global__ void reset_kernel(uint32_t *counter, unsigned long long *stats) {
     if (t < ST_N) stats[t] = 0;
 }
 
-// iS[k] = iA[32k]: a 32x thinner copy of the ion ids, for boundary searches
-__global__ void sample32_kernel(const uint32_t *__restrict__ iA, uint32_t *__restrict__ iS, uint64_t n) {
+// iS[k] = iA[k * kSampleStep]: a thinner copy of the ion ids, for the plan's
+// boundary brackets
+__global__ void sample_ids_kernel(const uint32_t *__restrict__ iA, uint32_t *__restrict__ iS, uint64_t n) {
     for (uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += (uint64_t)gridDim.x * blockDim.x)
-        iS[k] = iA[32 * k];
+        iS[k] = iA[k << kSampleShift];
 }
 
 __global__ void ion_bin_kernel(const uint32_t *__restrict__ bA, const uint32_t *__restrict__ iA,
@@ -430,12 +431,12 @@ hipError_t launch_search(const SearchArgs &a, int n_blocks, hipStream_t s) {
     return hipGetLastError();
 }
 
-hipError_t launch_sample32(const uint32_t *iA, uint32_t *iS, uint64_t n_ions, hipStream_t s) {
-    const uint64_t n = (n_ions + 31) / 32;
+hipError_t launch_sample_ids(const uint32_t *iA, uint32_t *iS, uint64_t n_ions, hipStream_t s) {
+    const uint64_t n = (n_ions + kSampleStep - 1) >> kSampleShift;
     if (n == 0) return hipSuccess;
     uint64_t blocks = (n + 255) / 256;
     if (blocks > 65535) blocks = 65535;
-    hipLaunchKernelGGL(sample32_kernel, dim3((unsigned)blocks), dim3(256), 0, s, iA, iS, n);
+    hipLaunchKernelGGL(sample_ids_kernel, dim3((unsigned)blocks), dim3(256), 0, s, iA, iS, n);
     return hipGetLastError();
 }
 

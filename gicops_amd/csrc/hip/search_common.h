@@ -16,7 +16,7 @@ struct DevBucket {
     const uint32_t *bA;
     const uint32_t *iA;     // n_entries * speclen ion ids, then kIaPad sentinels (0xffffffff)
     const uint32_t *ion_bin;
-    const uint32_t *iS;     // every 32nd ion id: iS[k] = iA[32k] (boundary searches)
+    const uint32_t *iS;     // every kSampleStep-th ion id: iS[k] = iA[k kSampleStep] (boundary brackets)
     uint32_t n_entries;
     uint32_t speclen;
     // exact division by speclen for n < 2^31: n / speclen == umulhi(n, div_m) >> div_s
@@ -59,6 +59,9 @@ struct SearchArgs {
 
 // Limits of the LDS-resident per-spectrum state.
 constexpr int kMaxPeaksLds = 2048;   // legal peaks per spectrum
+// the plan brackets tile boundaries from every kSampleStep-th ion id
+constexpr uint32_t kSampleShift = 4;
+constexpr uint32_t kSampleStep = 1u << kSampleShift;
 // iA is followed by kIaPad sentinel slots (0xffffffff, no ion id): the batched
 // search's lanes past a round's ions read them instead of branching
 constexpr uint32_t kIaPad = 1024;
@@ -73,7 +76,7 @@ constexpr int kStatRows = 64;        // stat counters are spread over rows to av
 
 hipError_t launch_reset(uint32_t *counter, unsigned long long *stats, hipStream_t s);
 hipError_t launch_search(const SearchArgs &a, int n_blocks, hipStream_t s);
-hipError_t launch_sample32(const uint32_t *iA, uint32_t *iS, uint64_t n_ions, hipStream_t s);
+hipError_t launch_sample_ids(const uint32_t *iA, uint32_t *iS, uint64_t n_ions, hipStream_t s);
 hipError_t launch_ion_bin(const uint32_t *bA, const uint32_t *iA, uint32_t *ion_bin, uint32_t nbins,
                           uint64_t n_ions, hipStream_t s);
 
