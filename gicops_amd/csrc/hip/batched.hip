@@ -378,17 +378,18 @@ __device__ __forceinline__ unsigned long long chunk_bit(uint32_t f) {
 // ---------------------------------------------------------------------------
 template <int THREADS, int TILE_WORDS, int SL>
 struct SpecLds {
-    alignas(16) uint32_t cnt[TILE_WORDS];  // count tile; pass-2 accumulators and the peak index after pass 1
     // non-empty slices of the tile: first ion minus the slice's flat offset, as
     // a biased byte offset (an ion position in the WIDE instance); entry ne is
     // the virtual slice past the last one and ne + 1 the round's dummy slot,
-    // both of which lead into iA's padding
+    // both of which lead into iA's padding. First in LDS, so that its byte
+    // addresses (chunk base + 4 * lane's slice) need no separate base register.
     uint32_t s_a[SL + 2];
+    alignas(16) uint32_t cnt[TILE_WORDS];  // count tile; pass-2 accumulators and the peak index after pass 1
     uint32_t s_flat[SL + 1];   // exclusive prefix of their lengths
     uint32_t dummy[64];        // per-lane word that out-of-tile lanes add 0 to
     alignas(16) uint32_t wsum[THREADS / 64];   // per-wave partial sums (read back as 16-byte vectors)
     alignas(16) uint32_t wsum2[THREADS / 64];
-    uint16_t cj0[THREADS];     // slices starting before each chunk of the round
+    uint16_t cj0[THREADS];     // per chunk of the round: 4 x (index of the slice at its lane 0)
     unsigned long long cmask[THREADS];  // lanes of each chunk where a slice starts
     uint32_t ne;
     uint32_t cand[kCandLdsSpec];  // the spectrum's candidates, b << 26 | psid (the rest in global scratch)
@@ -601,9 +602,11 @@ __global__ void __launch_bounds__(THREADS, WAVES_PER_SIMD) search_spectra_kernel
     for (int w = tid; w < TILE_WORDS; w += THREADS) L.cnt[w] = 0;
     for (int w = tid; w < GX_HIST_BINS; w += THREADS) L.hist[w] = 0;
     L.cmask[tid] = 0;
-    // never changed (out-of-tile lanes add 0): ~(min_cnt - 1), whose low field,
-    // the one such a lane's check reads (its el is 0), never equals min_cnt - 1
-    if (tid < 64) L.dummy[tid] = ~(min_cnt - 1u);
+    // never changed (out-of-tile lanes add 0). Such a lane's check reads the
+    // field its (meaningless) entry index selects, at bit 0, 8, 16 or 24 with
+    // width 8, 16 or 31; every byte of the word is (min_cnt - 1) ^ 1 in its low
+    // 8 bits, so no such field equals min_cnt - 1.
+    if (tid < 64) L.dummy[tid] = (((min_cnt - 1u) ^ 1u) & 0xffu) * 0x01010101u;
     for (int b = tid; b < a.n_buckets; b += THREADS) L.b_speclen[b] = a.buckets[b].speclen;
     const uint32_t nunits = a.n_units;
     if (tid == 0) {
@@ -679,7 +682,7 @@ __global__ void __launch_bounds__(THREADS, WAVES_PER_SIMD) search_spectra_kernel
             const uint32_t tiles = (uint32_t)__builtin_amdgcn_readlane((int)w_tiles, (int)b);
             const int32_t lo = __builtin_amdgcn_readlane(w_lo, (int)b), hi = __builtin_amdgcn_readlane(w_hi, (int)b);
             const uint32_t pack = (uint32_t)__builtin_amdgcn_readlane((int)w_pack, (int)b);
-            const uint32_t lp = pack == 4 ? 2u : (pack == 2 ? 1u : 0u);  // log2 entries per word
+            const uint32_t lp = (uint32_t)__builtin_ctz(pack);  // log2 entries per word (pack is 1, 2 or 4)
             const uint32_t fbits = 32u >> lp;                             // counter width
             const uint32_t fbw = fbits == 32 ? 31u : fbits;               // field width for ubfe (counts < 2^31)
             const uint32_t fsh = 5u - lp;                                 // log2 of the counter width
@@ -808,7 +811,8 @@ __global__ void __launch_bounds__(THREADS, WAVES_PER_SIMD) search_spectra_kernel
                             lds_barrier();
                         }
                         {
-                            const uint32_t v = (uint32_t)tid < nr ? (uint32_t)__popcll(L.cmask[tid]) : 0u;
+                            const unsigned long long cm = (uint32_t)tid < nr ? L.cmask[tid] : 0ull;
+                            const uint32_t v = (uint32_t)__popcll(cm);
                             const uint32_t inc = wave_incl_scan(v, lane);
                             if (lane == 63) L.wsum[wave] = inc;
                             lds_barrier();
@@ -820,7 +824,11 @@ __global__ void __launch_bounds__(THREADS, WAVES_PER_SIMD) search_spectra_kernel
                                       (4 * g + 2 < wave ? x.z : 0u) + (4 * g + 3 < wave ? x.w : 0u);
                                 tot += (x.x + x.y) + (x.z + x.w);
                             }
-                            if ((uint32_t)tid < nr) L.cj0[tid] = (uint16_t)(base + wb + inc - v);
+                            // a lane's slice in chunk c is jb + (slice starts at lanes 1..L of
+                            // the chunk), with jb = (slices starting before the chunk) - 1, + 1
+                            // when one starts at lane 0 (bit 63); kept as the byte offset 4 jb
+                            if ((uint32_t)tid < nr)
+                                L.cj0[tid] = (uint16_t)((base + wb + inc - v + (uint32_t)(cm >> 63) - 1u) << 2);
                             // the dummy slot: chunks past the round read iA's padding
                             if (tid == 0) L.s_a[ne + 1] = ((sent - ((r0 + nr) << 6)) << so_shift) + so_bias;
                             base += tot;
@@ -839,7 +847,7 @@ __global__ void __launch_bounds__(THREADS, WAVES_PER_SIMD) search_spectra_kernel
                                 // round or step (their mask writes come after at least one
                                 // barrier); the wave takes the metadata from that lane. A chunk
                                 // past the round is a dummy whose lanes all read the padding.
-                                uint32_t my_j0 = ne + 1;
+                                uint32_t my_j0 = (ne + 1) << 2;
                                 unsigned long long my_mk = 1ull << 63;
                                 if (lane < U && c + (uint32_t)lane < nr) {
                                     const uint32_t cc = c + (uint32_t)lane;
@@ -857,13 +865,11 @@ __global__ void __launch_bounds__(THREADS, WAVES_PER_SIMD) search_spectra_kernel
                             }
 #pragma unroll
                             for (int u = 0; u < U; u++) {
-                                // slices starting at or before this lane, minus one: a slice
-                                // starting at lane 0 (bit 63) counts for every lane, one at
-                                // lane k >= 1 (bit k - 1) for lanes >= k, i.e. bits below the lane
-                                const uint32_t jb = j0[u] + (uint32_t)(mk[u] >> 63) - 1u;
+                                // slice starts at lanes 1..L of the chunk (bit k - 1 for lane k;
+                                // bit 63, lane 0, is never below a lane)
                                 const uint32_t j = __builtin_amdgcn_mbcnt_hi(
-                                    (uint32_t)(mk[u] >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)mk[u], jb));
-                                sa[u] = L.s_a[j];
+                                    (uint32_t)(mk[u] >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)mk[u], 0u));
+                                sa[u] = *(const uint32_t *)((const char *)L.s_a + (j0[u] + (j << 2)));
                             }
                             // every lane loads, with no select: a lane past the round's
                             // ions reads the padding (no ion id), dropped with the ids
@@ -881,7 +887,7 @@ __global__ void __launch_bounds__(THREADS, WAVES_PER_SIMD) search_spectra_kernel
                             // the batch's counter increments are issued together, without
                             // a branch: a lane outside the tile adds 0 to its dummy word.
                             // One wait then covers all their returned old values.
-                            uint32_t old[U], elv[U];
+                            uint32_t old[U], xv[U];
 #pragma unroll
                             for (int u = 0; u < U; u++) {
                                 // ids of the tile's entries are [tlo_id, tlo_id + span_id);
@@ -893,18 +899,28 @@ __global__ void __launch_bounds__(THREADS, WAVES_PER_SIMD) search_spectra_kernel
                                 n_scanned += (uint32_t)__popcll(__ballot(in_tile));
                                 // floor(raw / speclen) - tlo, as tlo_id = tlo * speclen
                                 const uint32_t el = __umulhi(dr, dm) >> dsh;
-                                const uint32_t sh = (el & (pack - 1)) << fsh;
+                                // the counter's bit offset in its word is x mod 32 (shifts and
+                                // bit-field extracts read the low 5 bits of the offset); x >> fsh
+                                // gives el back
+                                const uint32_t x = el << fsh;
                                 uint32_t *w = in_tile ? &L.cnt[el >> lp] : &L.dummy[lane];
-                                old[u] = atomicAdd(w, in_tile ? 1u << sh : 0u);
-                                elv[u] = in_tile ? el : 0u;
+                                const uint32_t one = 1u << (x & 31u);
+                                old[u] = atomicAdd(w, in_tile ? one : 0u);
+                                xv[u] = x;
                             }
+                            // the increment that reaches min_shp makes the entry a candidate,
+                            // exactly once. One uniform test for the batch; the rare batch
+                            // that makes candidates tests its chunks again (the empty asm
+                            // keeps the compiler from holding 12 lane masks for that)
+                            unsigned long long hits = 0;
+#pragma unroll
+                            for (int u = 0; u < U; u++) hits |= __ballot(__builtin_amdgcn_ubfe(old[u], xv[u], fbw) == min_cnt - 1u);
+                            if (hits)
 #pragma unroll
                             for (int u = 0; u < U; u++) {
-                                // the increment that reaches min_shp makes the entry a
-                                // candidate, exactly once
-                                const uint32_t el = elv[u];
-                                const uint32_t sh = (el & (pack - 1)) << fsh;
-                                if (__builtin_amdgcn_ubfe(old[u], sh, fbw) == min_cnt - 1u) {
+                                asm volatile("" : "+v"(old[u]));
+                                if (__builtin_amdgcn_ubfe(old[u], xv[u], fbw) == min_cnt - 1u) {
+                                    const uint32_t el = xv[u] >> fsh;
                                     const uint32_t p = atomicAdd(&L.n_cand, 1u);
                                     const uint32_t cd = btag | ((uint32_t)tlo + el);
                                     if (p < (uint32_t)kCandLdsSpec)
