@@ -89,7 +89,7 @@ struct ItemArgs {
     int wide;                // 1: ion stream by 64-bit addresses (the WIDE kernel instance)
     uint32_t max_speclen;    // pass-2 task stride, with its exact divisor
     uint32_t smax_m, smax_s;
-    uint32_t cell_shift;     // bins per coarse peak-index cell = 2^cell_shift
+    uint32_t cell_shift;     // bins per pass-2 cell-table cell = 2^cell_shift
     // per-spectrum results
     uint32_t *hist;          // n_spectra * GX_HIST_BINS
     uint32_t *cpsms;

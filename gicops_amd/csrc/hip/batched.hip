@@ -431,24 +431,25 @@ __device__ __forceinline__ void merge_best(unsigned long long &k, uint32_t &e, u
 // peaks, flattened over (candidate, fragment) run by run (a run's candidates
 // share a bucket, so its stride is that bucket's speclen) and summed with LDS
 // atomics; then hyperscore, histogram and top hit. Peaks near a bin are found
-// through a coarse index (first peak of each 2^cell_shift-bin cell) kept in
-// the tail of the count tile.
+// through a cell table (first peak of each 2^cell_shift-bin cell and its bin)
+// kept in the tail of the count tile: most bins have no peak within dF and are
+// ruled out by one table read; the others walk the sorted peaks, the lane's U
+// tasks one peak per step together.
 template <int THREADS, int TILE_WORDS, int SL>
 __device__ void rescore_candidates(const ItemArgs &a, SpecLds<THREADS, TILE_WORDS, SL> &L, int P, uint32_t qoff,
                                    uint32_t *g_cand, uint32_t &n_err) {
-    // the tile during pass 2 (it is clear between items): accumulators, the
-    // spectrum's peaks (m/z bins and intensities, kPeakCap each, copied here
-    // per call), the coarse peak index
-    constexpr int kAccWords = TILE_WORDS - kPeakIdxWords - 2 * kPeakCap;
+    // the tile during pass 2 (it is clear between items): per candidate of a
+    // round three accumulators and its word; the spectrum's peaks (copied here
+    // per call) and a sentinel; the cell table
+    constexpr int kAccWords = TILE_WORDS - kPeakIdxWords - 2 * (kPeakCap + 1);
     static_assert(kAccWords > 0, "count tile too small for pass 2");
-    constexpr int CB = kAccWords / 3;  // candidates per round (3 accumulators each)
+    constexpr int CB = kAccWords / 4;  // candidates per round (3 accumulators and the word each)
     const int tid = threadIdx.x;
-    int32_t *pk_q = reinterpret_cast<int32_t *>(L.cnt + kAccWords);
-    int32_t *pk_i = pk_q + kPeakCap;
-    for (int k = tid; k < P; k += THREADS) {
-        pk_q[k] = a.lq[qoff + k];
-        pk_i[k] = a.li[qoff + k];
-    }
+    // the peaks as (m/z bin, intensity) pairs, P first, a sentinel (bin past every
+    // fragment) at P
+    int2 *pk = reinterpret_cast<int2 *>(L.cnt + kAccWords);
+    for (int k = tid; k <= P; k += THREADS)
+        pk[k] = k < P ? make_int2(a.lq[qoff + k], a.li[qoff + k]) : make_int2(0x7fffffff, 0);
     lds_barrier();
     // this call's top hit and candidate count; merged into the wave's slot at
     // the end, so that nothing of it stays live in registers between calls
@@ -456,15 +457,38 @@ __device__ void rescore_candidates(const ItemArgs &a, SpecLds<THREADS, TILE_WORD
     uint32_t beq = 0, bshp = 0, bcps = 0;
     const int dF = a.dF;
     const uint32_t ncand = L.n_cand;
-    uint16_t *first = reinterpret_cast<uint16_t *>(L.cnt + (TILE_WORDS - kPeakIdxWords));
+    // per 2^cs-bin cell: the first peak at or after the cell's start, as its index
+    // << 16 | its bin's offset from the start (0xffff: 65535 bins or more, or none)
+    uint32_t *cell_tab = L.cnt + (TILE_WORDS - kPeakIdxWords);
     const uint32_t cs = a.cell_shift;
     const uint32_t ncell = ((uint32_t)a.nbins >> cs) + 1;
-    for (uint32_t c = tid; c < ncell; c += THREADS) first[c] = (uint16_t)lower_bound_lds(pk_q, P, (int32_t)(c << cs));
+    for (uint32_t c = tid; c < ncell; c += THREADS) {
+        int lo = 0, n = P;
+        const int32_t v = (int32_t)(c << cs);
+        while (n > 0) {
+            const int h = n >> 1;
+            if (pk[lo + h].x < v) {
+                lo += h + 1;
+                n -= h + 1;
+            } else {
+                n = h;
+            }
+        }
+        const uint32_t off = lo < P ? (uint32_t)min(pk[lo].x - v, 0xffff) : 0xffffu;
+        cell_tab[c] = ((uint32_t)lo << 16) | off;
+    }
     uint32_t *acc_n = L.cnt, *acc_ib = L.cnt + CB, *acc_iy = L.cnt + 2 * CB;
     const uint32_t nseg = (uint32_t)__builtin_amdgcn_readfirstlane((int)L.n_seg);
     lds_barrier();
+    uint32_t *cw = L.cnt + 3 * CB;
     for (uint32_t c0 = 0; c0 < ncand; c0 += CB) {
         const uint32_t nc = min((uint32_t)CB, ncand - c0);
+        // the round's candidate words, from LDS or global scratch
+        for (uint32_t c = tid; c < nc; c += THREADS) {
+            const uint32_t ci = c0 + c;
+            cw[c] = ci < (uint32_t)kCandLdsSpec ? L.cand[ci] : gptr(g_cand)[ci - kCandLdsSpec];
+        }
+        lds_barrier();
         for (uint32_t k = 0; k < nseg; k++) {
             // the run's candidates in this round, (candidate, fragment) tasks
             // with the run's bucket's speclen as stride
@@ -479,31 +503,16 @@ __device__ void rescore_candidates(const ItemArgs &a, SpecLds<THREADS, TILE_WORD
             const auto ib = gptr(bk.ion_bin);
             const uint32_t nrun = s1 - s0, tasks = nrun * spl;  // < 2^31: the division below is exact
             constexpr int U = 4;
-            // the candidate words of the next iteration's tasks are loaded one
-            // iteration ahead (most sit in global scratch when a spectrum has many)
-            auto cand_of = [&](uint32_t tk) {
-                const uint32_t ci = s0 + min(__umulhi(tk, dm) >> dsh, nrun - 1);
-                return ci < (uint32_t)kCandLdsSpec ? L.cand[ci] : gptr(g_cand)[ci - kCandLdsSpec];
-            };
-            uint32_t cdn[U];
-#pragma unroll
-            for (int u = 0; u < U; u++) cdn[u] = cand_of((uint32_t)tid + (uint32_t)u * THREADS);
             for (uint32_t tb = tid; tb < tasks; tb += U * THREADS) {
                 int32_t x[U];
-                uint32_t cl[U], rr[U], cdc[U];
-#pragma unroll
-                for (int u = 0; u < U; u++) cdc[u] = cdn[u];
-                if (tb + U * THREADS < tasks) {
-#pragma unroll
-                    for (int u = 0; u < U; u++) cdn[u] = cand_of(tb + (uint32_t)(U + u) * THREADS);
-                }
+                uint32_t cl[U], rr[U];
 #pragma unroll
                 for (int u = 0; u < U; u++) {
                     const uint32_t tk = tb + (uint32_t)u * THREADS;
                     const uint32_t c = __umulhi(tk, dm) >> dsh;
                     const uint32_t r = tk - c * spl;
                     const uint32_t ci = s0 + min(c, nrun - 1);
-                    const uint32_t cd = cdc[u];
+                    const uint32_t cd = cw[ci - c0];
                     const bool ok = tk < tasks;
                     // every candidate of the run is of its bucket (a self-check)
                     n_err += ok && (cd >> 26) != b ? 1u : 0u;
@@ -513,21 +522,41 @@ __device__ void rescore_candidates(const ItemArgs &a, SpecLds<THREADS, TILE_WORD
                     // unused: cl marks it), so no select waits for the load
                     x[u] = (int32_t)ib[ok ? (size_t)(cd & kPsidMask) * spl + r : 0];
                 }
+                // peaks within +-dF of each fragment bin: the cell table rules out
+                // most bins at once; the rest walk the peaks from the cell's first,
+                // all U tasks one peak per step (a finished task reads the sentinel)
+                uint32_t kk[U], n[U], sm[U];
+                bool any = false;
 #pragma unroll
                 for (int u = 0; u < U; u++) {
-                    if (cl[u] == 0xffffffffu) continue;
-                    const int32_t lo = x[u] - dF, hi = x[u] + dF;
-                    int kk = first[(uint32_t)max(lo, 0) >> cs];
-                    while (kk < P && pk_q[kk] < lo) kk++;
-                    uint32_t n = 0, sum = 0;
-                    for (; kk < P && pk_q[kk] <= hi; kk++) {
-                        n++;
-                        sum += (uint32_t)pk_i[kk];
+                    const int32_t lo = x[u] - dF;
+                    const uint32_t c = (uint32_t)max(lo, 0) >> cs;
+                    const uint32_t e = cell_tab[c];
+                    const bool walk = cl[u] != 0xffffffffu && (int32_t)((c << cs) + (e & 0xffffu)) <= x[u] + dF;
+                    kk[u] = walk ? e >> 16 : (uint32_t)P;
+                    n[u] = 0;
+                    sm[u] = 0;
+                    any |= walk;
+                }
+                while (__ballot(any)) {
+                    any = false;
+#pragma unroll
+                    for (int u = 0; u < U; u++) {
+                        const int2 q = pk[kk[u]];
+                        const bool le = q.x <= x[u] + dF;
+                        const bool in = le && q.x >= x[u] - dF;
+                        n[u] += in ? 1u : 0u;
+                        sm[u] += in ? (uint32_t)q.y : 0u;
+                        kk[u] = le ? kk[u] + 1u : (uint32_t)P;
+                        any |= le;
                     }
-                    if (n) {
+                }
+#pragma unroll
+                for (int u = 0; u < U; u++) {
+                    if (n[u]) {
                         const bool isy = rr[u] >= (spl >> 1);
-                        atomicAdd(&acc_n[cl[u]], isy ? (n << 16) : n);
-                        atomicAdd(isy ? &acc_iy[cl[u]] : &acc_ib[cl[u]], sum);
+                        atomicAdd(&acc_n[cl[u]], isy ? (n[u] << 16) : n[u]);
+                        atomicAdd(isy ? &acc_iy[cl[u]] : &acc_ib[cl[u]], sm[u]);
                     }
                 }
             }
@@ -540,8 +569,8 @@ __device__ void rescore_candidates(const ItemArgs &a, SpecLds<THREADS, TILE_WORD
             acc_n[c] = 0;
             acc_ib[c] = 0;
             acc_iy[c] = 0;
-            const uint32_t ci = c0 + c;
-            const uint32_t cd = ci < (uint32_t)kCandLdsSpec ? L.cand[ci] : gptr(g_cand)[ci - kCandLdsSpec];
+            const uint32_t cd = cw[c];
+            cw[c] = 0;
             const uint32_t shpk = bc + yc;
             // pass 1 counted at least min_shp matches for this entry; the exact
             // recount must agree
@@ -558,11 +587,8 @@ __device__ void rescore_candidates(const ItemArgs &a, SpecLds<THREADS, TILE_WORD
         }
         lds_barrier();
     }
-    for (uint32_t c = tid; c < (ncell + 1) / 2; c += THREADS) L.cnt[TILE_WORDS - kPeakIdxWords + c] = 0;
-    for (int k = tid; k < P; k += THREADS) {
-        pk_q[k] = 0;
-        pk_i[k] = 0;
-    }
+    for (uint32_t c = tid; c < ncell; c += THREADS) cell_tab[c] = 0;
+    for (int k = tid; k <= P; k += THREADS) pk[k] = make_int2(0, 0);
     if (tid == 0) {
         L.n_cand = 0;
         L.n_seg = 0;

@@ -261,7 +261,7 @@ int index_create(gx_ctx *c, const gx_bucket *b, int nb, int maxz, uint32_t *cons
         d->h_buckets[i] = db;
     }
     if (!make_divisor(std::max(d->max_speclen, 2u), d->smax_m, d->smax_s)) return fail(GX_ERR_INVLD_INDEX);
-    while ((nbins >> d->cell_shift) + 1 > 2u * (uint32_t)kPeakIdxWords) d->cell_shift++;
+    while ((nbins >> d->cell_shift) + 1 > (uint32_t)kPeakIdxWords) d->cell_shift++;
     if (dalloc(&d->d_buckets, nb)) return fail(GX_ERR_BAD_MEM_ALLOC);
     if (hipMemcpy(d->d_buckets, d->h_buckets.data(), sizeof(DevBucket) * nb, hipMemcpyHostToDevice) != hipSuccess)
         return fail(GX_ERR_DEVICE);

@@ -52,7 +52,7 @@ struct gx_dindex {
     uint32_t max_speclen = 0;
     uint64_t max_ions = 0;            // largest bucket's ion count (picks the search's addressing)
     uint32_t smax_m = 0, smax_s = 0;  // exact division by max_speclen
-    uint32_t cell_shift = 7;         // coarse peak-index cell: 2^cell_shift bins
+    uint32_t cell_shift = 7;         // pass-2 cell table: 2^cell_shift bins per cell
     std::vector<void *> allocs;
     DevBucket *d_buckets = nullptr;
     std::vector<DevBucket> h_buckets;

@@ -71,7 +71,7 @@ constexpr int kTileWords = 11264;    // count tile: 2 entries (u16) per word -> 
 constexpr int kTileEntries = 2 * kTileWords;
 constexpr int kSliceLds = 1024;      // slices kept in LDS, more go to scratch
 constexpr int kCandLds = 1024;       // candidates kept in LDS, more go to scratch
-constexpr int kPeakIdxWords = 2048;  // coarse peak index of the batched pass 2 (u16 per cell, 4096 cells)
+constexpr int kPeakIdxWords = 2048;  // cell table of the batched pass 2 (one word per cell, 2048 cells)
 constexpr int kStatRows = 64;        // stat counters are spread over rows to avoid one hot address
 
 hipError_t launch_reset(uint32_t *counter, unsigned long long *stats, hipStream_t s);
