@@ -214,9 +214,11 @@ __global__ void __launch_bounds__(1024) scan_kernel(PlanArgs a) {
 // brackets of the first positions in [st, en) with iA[p] >= x0 and >= x1 from
 // the bin's samples iS (every kSampleStep-th id) alone: p0 in [lo0, hi0], p1 in
 // [lo1, hi1], each with hi - lo < kSampleStep. The two searches' loads are
-// independent, so each step waits for one latency instead of two.
+// independent, so each step waits for one latency instead of two. A search
+// turned off (do0 / do1 false) reads nothing; its bracket is then unused.
 __device__ __forceinline__ void bracket_sampled2(const DevBucket &bk, uint32_t st, uint32_t en, uint32_t x0, uint32_t x1,
-                                                 uint32_t &lo0, uint32_t &hi0, uint32_t &lo1, uint32_t &hi1) {
+                                                 uint32_t &lo0, uint32_t &hi0, uint32_t &lo1, uint32_t &hi1,
+                                                 bool do0 = true, bool do1 = true) {
     lo0 = st;
     hi0 = en;
     lo1 = st;
@@ -225,7 +227,7 @@ __device__ __forceinline__ void bracket_sampled2(const DevBucket &bk, uint32_t s
     if (en > 0 && k0 <= ((en - 1u) >> kSampleShift)) {
         const uint32_t k1 = (en - 1u) >> kSampleShift;
         const auto is = gptr(bk.iS);
-        uint32_t b0 = k0, b1 = k0, n0 = k1 - k0 + 1u, n1 = n0;
+        uint32_t b0 = k0, b1 = k0, n0 = do0 ? k1 - k0 + 1u : 0u, n1 = do1 ? k1 - k0 + 1u : 0u;
         while (n0 > 0 || n1 > 0) {
             const uint32_t h0 = n0 >> 1, h1 = n1 >> 1;
             const uint32_t v0 = n0 > 0 ? is[b0 + h0] : 0u, v1 = n1 > 0 ? is[b1 + h1] : 0u;
@@ -312,7 +314,7 @@ __global__ void __launch_bounds__(kSliceThreads) plan_slices_kernel(PlanArgs a) 
             whole = en - st <= a.whole_bin;
             if (!whole) {
                 uint32_t l0, h0, l1, h1;
-                bracket_sampled2(bk, st, en, lo * speclen, end_id, l0, h0, l1, h1);
+                bracket_sampled2(bk, st, en, lo * speclen, end_id, l0, h0, l1, h1, !from_start, !to_end);
                 c0 = from_start ? st : l0;
                 cz = to_end ? en : h1;
             }
