@@ -665,6 +665,14 @@ __global__ void __launch_bounds__(THREADS, WAVES_PER_SIMD) search_spectra_kernel
             L.w_cps[wave] = 0;
         }
         if (tid == 0) L.n_seg = 0;  // (the unit's candidate list is empty)
+        // quads of the count tile left to clear by the last item (cleared in the
+        // next item's setup, or before pass 2, which uses the tile's words)
+        uint32_t dirty_q = 0;
+        auto clear_tile = [&]() {
+            uint4 *cq = reinterpret_cast<uint4 *>(L.cnt);
+            for (uint32_t q = tid; q < dirty_q; q += THREADS) cq[q] = make_uint4(0, 0, 0, 0);
+            dirty_q = 0;
+        };
         uint32_t n_scanned = 0, n_errors = 0;
         const uint64_t pos_s = a.pos_off[s] - a.pos_off[a.chunk_s0];
         // the spectrum's per-bucket plan, bucket b in lane b of every wave (one
@@ -731,12 +739,14 @@ __global__ void __launch_bounds__(THREADS, WAVES_PER_SIMD) search_spectra_kernel
                 const uint32_t nent = (uint32_t)(thi - tlo + 1);
                 const uint32_t tlo_id = (uint32_t)tlo * speclen;
                 const uint32_t span_id = nent * speclen;
-                // room for every entry of this tile as a candidate? An LDS-only
-                // barrier, so that the next step's slice words loaded by the last
-                // step stay in flight; a full one before pass 2, which reads
-                // candidates that pass 1 wrote to global scratch
+                // the last item's stream is done (every wave's increments and
+                // candidate appends); room for every entry of this tile as a
+                // candidate? An LDS-only barrier, so that the next step's slice
+                // words loaded by the last step stay in flight; a full one before
+                // pass 2, which reads candidates that pass 1 wrote to global scratch
                 lds_barrier();
                 if (L.n_cand + nent > cand_cap) {
+                    clear_tile();
                     __syncthreads();
                     rescore_candidates<THREADS, TILE_WORDS, SL>(a, L, P, qoff, g_cand, n_errors);
                 }
@@ -785,6 +795,7 @@ __global__ void __launch_bounds__(THREADS, WAVES_PER_SIMD) search_spectra_kernel
                         L.wsum2[wave] = ifl;
                     }
                     lds_barrier();
+                    if (dirty_q) clear_tile();  // (ordered before the stream by the barrier below)
                     uint32_t bl = 0, bf = 0;
                     static_assert(NW % 4 == 0, "per-wave sums read four at a time");
 #pragma unroll
@@ -960,15 +971,12 @@ __global__ void __launch_bounds__(THREADS, WAVES_PER_SIMD) search_spectra_kernel
                         }
                     }
                 }
-                lds_barrier();
-                // ---- the candidates were collected in pass 1; clear the tile ----
-                {
-                    const uint32_t nq = (((nent + pack - 1) >> lp) + 3) >> 2;
-                    uint4 *cq = reinterpret_cast<uint4 *>(L.cnt);
-                    for (uint32_t q = tid; q < nq; q += THREADS) cq[q] = make_uint4(0, 0, 0, 0);
-                }
+                // the candidates were collected in pass 1; the tile is cleared later
+                dirty_q = (((nent + pack - 1) >> lp) + 3) >> 2;
             }
         }
+        lds_barrier();
+        clear_tile();
         __syncthreads();  // candidates in global scratch, see above
         if (L.n_cand) rescore_candidates<THREADS, TILE_WORDS, SL>(a, L, P, qoff, g_cand, n_errors);
         my_scanned += n_scanned;  // already the wave's total
