@@ -888,7 +888,7 @@ __global__ void __launch_bounds__(THREADS, WAVES_PER_SIMD) search_spectra_kernel
                                 // past the round is a dummy whose lanes all read the padding.
                                 uint32_t my_j0 = (ne + 1) << 2;
                                 unsigned long long my_mk = 1ull << 63;
-                                if (lane < U && c + (uint32_t)lane < nr) {
+                                if ((lane < U) & (c + (uint32_t)lane < nr)) {
                                     const uint32_t cc = c + (uint32_t)lane;
                                     my_j0 = L.cj0[cc];
                                     my_mk = L.cmask[cc];
@@ -948,13 +948,14 @@ __global__ void __launch_bounds__(THREADS, WAVES_PER_SIMD) search_spectra_kernel
                                 xv[u] = x;
                             }
                             // the increment that reaches min_shp makes the entry a candidate,
-                            // exactly once. One uniform test for the batch; the rare batch
-                            // that makes candidates tests its chunks again (the empty asm
-                            // keeps the compiler from holding 12 lane masks for that)
-                            unsigned long long hits = 0;
+                            // exactly once. One uniform test for the batch (a field equal to
+                            // min_shp - 1 makes the xor, and so the minimum, zero); the rare
+                            // batch that makes candidates tests its chunks again (the empty
+                            // asm keeps the compiler from holding 12 lane masks for that)
+                            uint32_t dmin = 0xffffffffu;
 #pragma unroll
-                            for (int u = 0; u < U; u++) hits |= __ballot(__builtin_amdgcn_ubfe(old[u], xv[u], fbw) == min_cnt - 1u);
-                            if (hits)
+                            for (int u = 0; u < U; u++) dmin = min(dmin, __builtin_amdgcn_ubfe(old[u], xv[u], fbw) ^ (min_cnt - 1u));
+                            if (__ballot(dmin == 0u))
 #pragma unroll
                             for (int u = 0; u < U; u++) {
                                 asm volatile("" : "+v"(old[u]));
