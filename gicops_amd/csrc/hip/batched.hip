@@ -400,6 +400,8 @@ struct SpecLds {
     uint32_t w_eq[THREADS / 64];
     uint32_t w_shp[THREADS / 64];
     uint32_t w_cps[THREADS / 64];
+    uint32_t w_scan[THREADS / 64];  // per wave: the unit's consumed ion ids and self-check errors
+    uint32_t w_err[THREADS / 64];
     uint32_t b_speclen[kMaxBuckets];
     // the candidates gathered since the last pass 2 come in runs, one per
     // bucket (items go bucket by bucket): run k starts at seg_start[k]
@@ -641,7 +643,6 @@ __global__ void __launch_bounds__(THREADS, WAVES_PER_SIMD) search_spectra_kernel
         L.next = atomicAdd(a.work_counter, 1u);
         L.n_cand = 0;
     }
-    unsigned long long my_cand = 0, my_scanned = 0, my_errors = 0;
 
     for (;;) {
         lds_barrier();
@@ -980,8 +981,13 @@ __global__ void __launch_bounds__(THREADS, WAVES_PER_SIMD) search_spectra_kernel
         clear_tile();
         __syncthreads();  // candidates in global scratch, see above
         if (L.n_cand) rescore_candidates<THREADS, TILE_WORDS, SL>(a, L, P, qoff, g_cand, n_errors);
-        my_scanned += n_scanned;  // already the wave's total
-        my_errors += n_errors;
+        {
+            const uint32_t werr = wave_sum_u32(n_errors);
+            if (lane == 0) {
+                L.w_scan[wave] = n_scanned;  // already the wave's total
+                L.w_err[wave] = werr;
+            }
+        }
         // ---- block reduction of the top hit; the unit's results ----
         lds_barrier();
         const bool whole = slot == kWholeUnit;
@@ -991,10 +997,12 @@ __global__ void __launch_bounds__(THREADS, WAVES_PER_SIMD) search_spectra_kernel
             L.hist[w] = 0;
         }
         if (tid == 0) {
-            unsigned long long key = 0;
+            unsigned long long key = 0, scanned = 0, errors = 0;
             uint32_t eq = 0, shp = 0, cps = 0;
             for (int w = 0; w < NW; w++) {
                 cps += L.w_cps[w];
+                scanned += L.w_scan[w];
+                errors += L.w_err[w];
                 merge_best(key, eq, shp, L.w_key[w], L.w_eq[w], L.w_shp[w]);
             }
             if (whole) {
@@ -1008,16 +1016,14 @@ __global__ void __launch_bounds__(THREADS, WAVES_PER_SIMD) search_spectra_kernel
                 a.part_shp[slot] = shp;
                 a.part_cps[slot] = cps;
             }
-            my_cand += cps;
+            // the unit's statistics (H, the window-restricted ion ids, is
+            // counted where they are consumed); per unit, so that no counter
+            // stays live in registers across the launch
+            stat_add(a.stats, ST_CAND, cps);
+            stat_add(a.stats, ST_IONS, scanned);
+            stat_add(a.stats, ST_SCANNED, scanned);
+            stat_add(a.stats, ST_ERRORS, errors);
         }
-    }
-    if (tid == 0) stat_add(a.stats, ST_CAND, my_cand);
-    my_errors = wave_sum_u64(my_errors);
-    if (lane == 0) {
-        // H (window-restricted ion ids) is counted where they are consumed
-        stat_add(a.stats, ST_IONS, my_scanned);
-        stat_add(a.stats, ST_SCANNED, my_scanned);
-        stat_add(a.stats, ST_ERRORS, my_errors);
     }
 }
 
