@@ -182,7 +182,8 @@ def judge_learned(psm, ref, st, raw) -> dict:
     with np.errstate(divide="ignore", invalid="ignore"):
         lr = np.log10(ref["evalue"][fit])
         ll = np.log10(psm["evalue"][fit])
-    ok[fit] = np.abs(ll - lr) <= 1e-3 * np.maximum(np.abs(lr), 1e-3)
+        # equal e-values count as within (an e-value of 0 or inf has no finite log10 difference)
+        ok[fit] = (psm["evalue"][fit] == ref["evalue"][fit]) | (np.abs(ll - lr) <= 1e-3 * np.maximum(np.abs(lr), 1e-3))
     ok[~fit] = ~sur[~fit]
     err = np.abs(ll - lr)[sur[fit]] if sur.any() else np.zeros(0)
     rel = (err / np.maximum(np.abs(lr[sur[fit]]), 1e-3)) if sur.any() else np.zeros(0)
