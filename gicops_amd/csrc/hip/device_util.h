@@ -197,18 +197,34 @@ __device__ __forceinline__ bool precursor_window(const DevBucket &bk, float prec
     const float pm2 = (float)((double)prec + dM);
     const auto mass = gptr(bk.mass);
     const float m0 = mass[0], mN = mass[n - 1];
-    if (pm1 < m0)
-        lo = 0;
-    else if (pm1 > mN)
-        return false;
-    else
-        lo = (int32_t)lower_bound_f32(mass, n, pm1);
-    if (pm2 > mN)
-        hi = (int32_t)n - 1;
-    else if (pm2 < m0)
-        return false;
-    else
-        hi = (int32_t)upper_bound_f32(mass, n, pm2) - 1;
+    if (!(pm1 < m0) && pm1 > mN) return false;
+    if (!(pm2 > mN) && pm2 < m0) return false;
+    // lower_bound of pm1 and upper_bound of pm2 in one loop, their loads
+    // independent (one latency per step instead of two)
+    const bool s0 = !(pm1 < m0), s1 = !(pm2 > mN);
+    uint32_t b0 = 0, n0 = s0 ? n : 0u, b1 = 0, n1 = s1 ? n : 0u;
+    while (n0 > 0 || n1 > 0) {
+        const uint32_t h0 = n0 >> 1, h1 = n1 >> 1;
+        const float v0 = n0 > 0 ? mass[b0 + h0] : 0.0f, v1 = n1 > 0 ? mass[b1 + h1] : 0.0f;
+        if (n0 > 0) {
+            if (v0 < pm1) {
+                b0 += h0 + 1;
+                n0 -= h0 + 1;
+            } else {
+                n0 = h0;
+            }
+        }
+        if (n1 > 0) {
+            if (!(pm2 < v1)) {
+                b1 += h1 + 1;
+                n1 -= h1 + 1;
+            } else {
+                n1 = h1;
+            }
+        }
+    }
+    lo = s0 ? (int32_t)b0 : 0;
+    hi = s1 ? (int32_t)b1 - 1 : (int32_t)n - 1;
     const bool ok = hi >= 0 && lo < (int32_t)n && mass[hi] <= pm2 && mass[lo] >= pm1 && hi >= lo;
     if (!ok) hi = lo - 1;
     return ok;
