@@ -30,6 +30,8 @@ namespace {
 constexpr int kPlanThreads = 256;
 constexpr int kSliceThreads = 128;  // plan_slices block (small blocks keep more of them in flight)
 constexpr int kPeakCap = kMaxPeaksLds;  // 2048 legal peaks per spectrum
+constexpr int kBitonicMin = 512;        // above this many legal peaks plan_spectra sorts them bitonically
+static_assert(kPeakCap <= 2048, "the bitonic sort keys hold an 11-bit peak index");
 constexpr int kStatSpread = kStatRows;
 constexpr int kUnroll = 6;              // stream chunks in flight per wave: 2 * kUnroll
 // The ion stream addresses iA as a 32-bit byte offset from a scalar base plus
@@ -94,16 +96,46 @@ __global__ void __launch_bounds__(kPlanThreads) plan_spectra_kernel(PlanArgs a) 
     }
     __syncthreads();
     const int P = (int)s_nlegal;
-    for (int k = tid; k < P; k += kPlanThreads) {
-        const int32_t q = tq[k];
-        int r = 0;
-        for (int j = 0; j < P; j++) {
-            const int32_t o = tq[j];
-            r += (o < q) || (o == q && j < k);
+    if (P > kBitonicMin && a.nbins <= (1 << 21)) {
+        // many peaks: a bitonic sort in LDS of (bin << 11 | compacted index),
+        // the index breaking ties as the rank sort below does
+        int np2 = kBitonicMin;
+        while (np2 < P) np2 <<= 1;
+        uint32_t *sk = reinterpret_cast<uint32_t *>(sq);
+        for (int k = tid; k < np2; k += kPlanThreads) sk[k] = k < P ? ((uint32_t)tq[k] << 11) | (uint32_t)k : 0xffffffffu;
+        __syncthreads();
+        for (int size = 2; size <= np2; size <<= 1) {
+            for (int stride = size >> 1; stride > 0; stride >>= 1) {
+                for (int i = tid; i < (np2 >> 1); i += kPlanThreads) {
+                    const int lo = ((i & ~(stride - 1)) << 1) | (i & (stride - 1)), hi = lo + stride;
+                    const uint32_t x = sk[lo], y = sk[hi];
+                    if ((x > y) == ((lo & size) == 0)) {
+                        sk[lo] = y;
+                        sk[hi] = x;
+                    }
+                }
+                __syncthreads();
+            }
         }
-        sq[r] = q;
-        a.lq[off + r] = q;
-        a.li[off + r] = ti[k];
+        for (int r = tid; r < P; r += kPlanThreads) {
+            const uint32_t key = sk[r];
+            const int32_t q = (int32_t)(key >> 11);
+            sq[r] = q;
+            a.lq[off + r] = q;
+            a.li[off + r] = ti[key & 2047u];
+        }
+    } else {
+        for (int k = tid; k < P; k += kPlanThreads) {
+            const int32_t q = tq[k];
+            int r = 0;
+            for (int j = 0; j < P; j++) {
+                const int32_t o = tq[j];
+                r += (o < q) || (o == q && j < k);
+            }
+            sq[r] = q;
+            a.lq[off + r] = q;
+            a.li[off + r] = ti[k];
+        }
     }
     __syncthreads();
     // most peaks any 2dF+1 consecutive bins hold: each fragment ion is counted
