@@ -418,7 +418,7 @@ struct SpecLds {
     // both of which lead into iA's padding. First in LDS, so that its byte
     // addresses (chunk base + 4 * lane's slice) need no separate base register.
     uint32_t s_a[SL + 2];
-    alignas(16) uint32_t cnt[TILE_WORDS];  // count tile; pass-2 accumulators and the peak index after pass 1
+    alignas(16) uint32_t cnt[TILE_WORDS];  // count tile; in pass 2 its accumulators, candidate words, peaks and cell table
     uint32_t s_flat[SL + 1];   // exclusive prefix of their lengths
     uint32_t dummy[64];        // per-lane word that out-of-tile lanes add 0 to
     alignas(16) uint32_t wsum[THREADS / 64];   // per-wave partial sums (read back as 16-byte vectors)
@@ -666,8 +666,9 @@ __global__ void __launch_bounds__(THREADS, WAVES_PER_SIMD) search_spectra_kernel
     L.cmask[tid] = 0;
     // never changed (out-of-tile lanes add 0). Such a lane's check reads the
     // field its (meaningless) entry index selects, at bit 0, 8, 16 or 24 with
-    // width 8, 16 or 31; every byte of the word is (min_cnt - 1) ^ 1 in its low
-    // 8 bits, so no such field equals min_cnt - 1.
+    // width 8, 16 or 31; every byte of the word is the low byte of
+    // (min_cnt - 1) ^ 1, so the low byte of any such field differs from that of
+    // min_cnt - 1 and no such field equals it.
     if (tid < 64) L.dummy[tid] = (((min_cnt - 1u) ^ 1u) & 0xffu) * 0x01010101u;
     for (int b = tid; b < a.n_buckets; b += THREADS) L.b_speclen[b] = a.buckets[b].speclen;
     const uint32_t nunits = a.n_units;
