@@ -98,6 +98,8 @@ def test_uneven_batch_parity(open_index, searcher_open, eng):
                          dm_choices=(10.0, 100.0, 500.0))
     batch = slm.generate_spectra(open_index, cfg)
     assert (np.diff(batch.idx) > 1024 // 5).any()
+    # spectra of more than 512 legal peaks: the batched plan sorts them with its bitonic network
+    assert (np.diff(batch.idx) > 600).sum() >= 10
     g, hg, st = searcher_open.search(batch, SearchParams(dM=500.0, **eng), want_hist=True)
     c, hc, cnt = O.search(open_index, batch, O.default_params(dM=500.0), want_hist=True, want_counters=True)
     check_stats(st, cnt, batch.n_spectra)
