@@ -334,8 +334,14 @@ __global__ void __launch_bounds__(kSliceThreads) plan_slices_kernel(PlanArgs a) 
         if (act) {
             const int k = j / nb5;
             const uint32_t bin = (uint32_t)(lq[k] - dF + (j - k * nb5));
-            st = gptr(bk.bA)[bin];
-            en = gptr(bk.bA)[bin + 1];
+            {
+                // the bin's start and end in one 8-byte load at a 4-byte aligned
+                // address (gfx9 global loads run in unaligned-access mode)
+                typedef unsigned long long __attribute__((aligned(4))) u64_a4;
+                const unsigned long long se = *(const gmem<u64_a4> *)(gptr(bk.bA) + bin);
+                st = (uint32_t)se;
+                en = (uint32_t)(se >> 32);
+            }
             bins++;
             // bins of at most whole_bin ions are streamed whole by the search
             // kernel (every tile filters by id). Larger ones get rows bracketed
