@@ -32,6 +32,7 @@ struct PlanArgs {
     double dM;
     int tile_words;          // count-tile words per item (the configuration's tile / split)
     uint32_t whole_bin;      // bins of at most this many ions are streamed whole
+    int slice_threads;       // plan_slices block size: 256 for an index with a bucket of more than 2^22 entries, else 128
     // outputs of plan_spectra
     int32_t *lq;        // sorted legal peaks (CSR at qidx offsets)
     int32_t *li;

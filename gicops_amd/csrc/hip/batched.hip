@@ -28,7 +28,9 @@ namespace gx {
 namespace {
 
 constexpr int kPlanThreads = 256;
-constexpr int kSliceThreads = 128;  // plan_slices block (small blocks keep more of them in flight)
+// plan_slices block: 128 threads keep more blocks in flight; 256 when the
+// index's buckets are large (longer windows, more tile boundaries per slice)
+constexpr int kSliceThreads = 128, kSliceThreadsLarge = 256;
 constexpr int kPeakCap = kMaxPeaksLds;  // 2048 legal peaks per spectrum
 constexpr int kBitonicMin = 512;        // above this many legal peaks plan_spectra sorts them bitonically
 static_assert(kPeakCap <= 2048, "the bitonic sort keys hold an 11-bit peak index");
@@ -290,7 +292,8 @@ __device__ __forceinline__ void bracket_sampled2(const DevBucket &bk, uint32_t s
 // one block per (bucket, spectrum) of the chunk, bucket-major and by precursor
 // mass within a bucket: the blocks in flight read one bucket's bA and iA with
 // similar windows, which stay in L2
-__global__ void __launch_bounds__(kSliceThreads) plan_slices_kernel(PlanArgs a) {
+template <int NT>
+__global__ void __launch_bounds__(NT) plan_slices_kernel(PlanArgs a) {
     __shared__ unsigned long long s_st[2];
     const int tid = threadIdx.x, lane = tid & 63;
     const uint32_t nb = (uint32_t)a.n_buckets;
@@ -326,7 +329,7 @@ __global__ void __launch_bounds__(kSliceThreads) plan_slices_kernel(PlanArgs a) 
     // order: the search does not depend on it); their number goes to pair_nsl.
     uint32_t *out = a.pos + base;
     const unsigned long long below = lane ? ((1ull << (uint32_t)lane) - 1ull) : 0ull;
-    for (int j0 = 0; j0 < nsl; j0 += kSliceThreads) {
+    for (int j0 = 0; j0 < nsl; j0 += NT) {
         const int j = j0 + tid;
         const bool act = j < nsl;
         uint32_t st = 0, en = 0, c0 = 0, cz = 0;
@@ -1169,7 +1172,10 @@ hipError_t launch_scan(const PlanArgs &a, hipStream_t s) {
 hipError_t launch_plan_slices(const PlanArgs &a, hipStream_t s) {
     const uint64_t nblk = (uint64_t)(a.chunk_s1 - a.chunk_s0) * (uint64_t)a.n_buckets;
     if (nblk == 0) return hipSuccess;
-    hipLaunchKernelGGL(plan_slices_kernel, dim3((unsigned)nblk), dim3(kSliceThreads), 0, s, a);
+    if (a.slice_threads == kSliceThreadsLarge)
+        hipLaunchKernelGGL(plan_slices_kernel<kSliceThreadsLarge>, dim3((unsigned)nblk), dim3(kSliceThreadsLarge), 0, s, a);
+    else
+        hipLaunchKernelGGL(plan_slices_kernel<kSliceThreads>, dim3((unsigned)nblk), dim3(kSliceThreads), 0, s, a);
     return hipGetLastError();
 }
 

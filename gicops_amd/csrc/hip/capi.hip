@@ -505,6 +505,13 @@ Batched setup_batched(gx_ctx *c, const gx_dindex *idx, gx_dqueries *q, const gx_
     pa.dM = p->dM;
     pa.tile_words = B.cfg.tile_words / B.split;
     pa.whole_bin = (uint32_t)(p->whole_bin > 0 ? p->whole_bin : c->tuned_whole);
+    {
+        // larger blocks plan faster on indexes with large buckets (measured: 8 % on
+        // a 244M-entry index) and slower on small ones (27 % on a 5.6M-entry one)
+        uint32_t max_ent = 0;
+        for (const DevBucket &bk : idx->h_buckets) max_ent = std::max(max_ent, bk.n_entries);
+        pa.slice_threads = max_ent > (1u << 22) ? 256 : 128;
+    }
     pa.lq = q->lq;
     pa.li = q->li;
     pa.n_legal = q->n_legal;
